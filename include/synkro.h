@@ -1,0 +1,120 @@
+/*
+ * synkro.h -- C ABI of libsynkro.so, the MI355X (gfx950) projector / SIRT library.
+ *
+ * This is the drop-in boundary for tomograph/synkrotomo's projector and SIRT path.  The
+ * reference exposes each Futhark entry point as a generated PyOpenCL class with one
+ * `main` method (Makefile:1-4, README.md:21-25); this library exposes the same
+ * operations as plain C functions over device pointers, and synkrotomo_amd/futhark/
+ * rebuilds the reference's module/class/`main` surface on top of it with ctypes.
+ *
+ * Conventions
+ *   - Device pointers (d_*) are caller-owned (normally torch tensors); the library never
+ *     frees them.  Host pointers (*_host) are read during the call only.
+ *   - Image:    float32 [N][N], row-major (row = y + N/2, column = x + N/2).
+ *   - Sinogram: float32 [A][R], angle-major, bin r <-> rho = rho0 + r*drho.
+ *   - Every launch goes on the caller's stream (hipStream_t passed as void*; NULL = the
+ *     null stream).  Calls return when the work is enqueued, except where noted.
+ *   - Return value: SK_OK, or an error code; sk_last_error() gives the message.
+ *   - One sk_ctx per host thread (or external locking): a context owns scratch memory
+ *     that its calls share.
+ */
+#ifndef SYNKRO_H
+#define SYNKRO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sk_ctx sk_ctx;
+typedef struct sk_geom sk_geom;
+typedef struct sk_mat sk_mat;
+
+enum {
+  SK_OK = 0,
+  SK_EINVAL = 1, /* bad argument value or shape (Python: ValueError) */
+  SK_ETYPE = 2,  /* bad argument kind (Python: TypeError) */
+  SK_EHIP = 3,   /* HIP runtime error (Python: RuntimeError) */
+  SK_ENOMEM = 4  /* device allocation failed (Python: MemoryError) */
+};
+
+/* Options for sk_set_option. */
+enum {
+  /* 1: every division in the inner loops is an IEEE division, so results are bitwise
+   *    those of the sequential FP32 restatement of the reference (slower).
+   * 0 (default): the per-step quotients use the hardware reciprocal (<= 2 ulp per
+   *    length); pixel indices and candidate sets are unchanged. */
+  SK_OPT_EXACT = 1,
+  /* Tile / rays-per-thread variant for the projector kernels (0 = autotuned default). */
+  SK_OPT_FP_VARIANT = 2,
+  SK_OPT_BP_VARIANT = 3
+};
+
+/* Operators that can be made explicit by sk_sparse_build. */
+enum {
+  SK_MAT_FP = 0, /* A: rows = rays (angle-major), columns = pixels */
+  SK_MAT_BP = 1  /* B: rows = pixels, columns = sinogram entries (the backprojector) */
+};
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart (replaces the generated class constructors, which create an
+ * OpenCL context and queue; Futhark convention, absent from the reference tree). */
+int sk_create(int device, sk_ctx** out);
+void sk_destroy(sk_ctx* ctx);
+const char* sk_last_error(const sk_ctx* ctx);
+int sk_set_option(sk_ctx* ctx, int option, int value);
+int sk_get_option(const sk_ctx* ctx, int option, int* value);
+
+/* ---------------------------------------------------------------- geometry */
+/* Replaces sirtlib.preprocess (futhark/sirtlib.fut:28-44), which every `main` runs on the
+ * device: per-angle cos/sin, line lengths, steep/flat split.  Built once on the host and
+ * uploaded.  Rejects A < 1, R < 1, odd or non-positive N, drho <= 0. */
+int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, float drho, int R,
+                   int N, sk_geom** out);
+void sk_geom_destroy(sk_geom* g);
+/* n_steep = number of angles with |sin| < |cos|; ncand = ceil(sqrt(2)/drho). */
+int sk_geom_query(const sk_geom* g, int* A, int* R, int* N, int* n_steep, int* ncand);
+
+/* Host only (no device needed): the line table sk_geom_create builds, 8 floats per line in
+ * steep-then-flat order: cost, sint, lbase, slope, fact*cost, fact*sint, orig, flags.
+ * Returns the number of steep lines (sirtlib.fut:28-44), or -1 on bad arguments. */
+int sk_lines_compute(const float* angles_host, int A, float* table_out);
+
+/* ---------------------------------------------------------------- dense projector */
+/* A.x: forwardprojection.main (futhark/forwardprojection.fut:71-84).
+ * d_img [batch][N][N] -> d_sino [batch][A][R]. */
+int sk_fp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_img, float* d_sino, void* stream);
+
+/* B.r: backprojection.main (futhark/backprojection.fut:82-93).
+ * d_sino [batch][A][R] -> d_img [batch][N][N]. */
+int sk_bp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img, void* stream);
+
+/* A^T.r, the exact adjoint of sk_fp (same per-entry lengths as A).  No reference
+ * counterpart: the reference's backprojection equals A^T except on lines whose slope
+ * rounds to zero (theta = 0, pi/2), where it drops rays lying on pixel edges.  Used as
+ * the backward of the differentiable projector. */
+int sk_bp_adjoint(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img,
+                  void* stream);
+
+/* ---------------------------------------------------------------- SIRT */
+/* SIRT.main (futhark/SIRT.fut:9-49): x <- x + C*B(R*(b - A x)) `iters` times, in place
+ * on d_img [batch][N][N], with R = 1/rowsum(A), C = 1/colsum(B) (0 where the sum is 0).
+ * The iteration (two fused kernels) is captured once into a hipGraph and replayed.
+ * batch > 1 runs independent slices of one geometry (SIRT3D.main,
+ * futhark/SIRT3D.fut:18-28). */
+int sk_sirt(sk_ctx* ctx, sk_geom* g, int batch, float* d_img, const float* d_sino, int iters,
+            void* stream);
+
+/* The SIRT weights: d_rowinv [A][R] = 1/rowsum, d_colinv [N][N] = 1/colsum (SIRT.fut:22-25). */
+int sk_sirt_weights(sk_ctx* ctx, sk_geom* g, float* d_rowinv, float* d_colinv, void* stream);
+
+/* One SIRT iteration with caller-provided weights (no graph, no state):
+ * d_q [batch][A][R] is scratch.  Used by the learned/unrolled paths. */
+int sk_sirt_step(sk_ctx* ctx, sk_geom* g, int batch, float* d_img, const float* d_sino,
+                 const float* d_rowinv, const float* d_colinv, float* d_q, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYNKRO_H */

@@ -1,0 +1,89 @@
+"""ctypes binding of libsynkro.so (include/synkro.h).
+
+The library is the only compute path: if it is missing or the device is absent, the
+functions here raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libsynkro.so")
+
+SK_OK, SK_EINVAL, SK_ETYPE, SK_EHIP, SK_ENOMEM = 0, 1, 2, 3, 4
+SK_OPT_EXACT, SK_OPT_FP_VARIANT, SK_OPT_BP_VARIANT = 1, 2, 3
+SK_MAT_FP, SK_MAT_BP = 0, 1
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# name -> (restype, argtypes); every symbol declared in include/synkro.h
+SIGNATURES = {
+    "sk_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "sk_destroy": (None, [_vp]),
+    "sk_last_error": (ctypes.c_char_p, [_vp]),
+    "sk_set_option": (_i, [_vp, _i, _i]),
+    "sk_get_option": (_i, [_vp, _i, _ip]),
+    "sk_geom_create": (_i, [_vp, _f32p, _i, _f, _f, _i, _i, ctypes.POINTER(_vp)]),
+    "sk_geom_destroy": (None, [_vp]),
+    "sk_geom_query": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    "sk_lines_compute": (_i, [_f32p, _i, _f32p]),
+    "sk_fp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_bp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_bp_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_sirt": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
+    "sk_sirt_weights": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "sk_sirt_step": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+
+class NativeMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libsynkro.so once and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeMissing(
+                f"{LIB_PATH} is not built; run `python -m synkrotomo_amd.build` "
+                "(the HIP library is the only compute path)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    """Map a C-ABI status to the Python exception the reference's wrapper would raise."""
+    if rc == SK_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        raw = lib().sk_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    text = f"{what}: {msg}" if what else msg
+    if rc == SK_EINVAL:
+        raise ValueError(text)
+    if rc == SK_ETYPE:
+        raise TypeError(text)
+    if rc == SK_ENOMEM:
+        raise MemoryError(text)
+    raise RuntimeError(text or f"libsynkro error {rc}")

@@ -1,0 +1,109 @@
+// sk_common.h -- internal declarations shared by the libsynkro translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/synkro.h"
+
+namespace sk {
+
+// Flags of a frame line.
+enum : int {
+  LINE_FLAT = 1,      // flat angle: the line acts on the transposed image
+  LINE_SLOPE_POS = 2, // slope >= 0
+  LINE_DEGEN = 4      // |slope| < 1e-3: x_top may round onto x_bot (theta ~ 0, pi/2)
+};
+
+// One line in its frame (futhark/sirtlib.fut:39-41): steep angles keep the image frame
+// with (cost, sint) = (cos, -sin); flat angles use the transposed frame with
+// (cost, sint) = (-sin, cos).  32 bytes, read with scalar loads by the kernels.
+struct alignas(32) Line {
+  float cost, sint, lbase, slope;  // slope = -sint/cost (forwardprojection.fut:29)
+  float fc, fs;                    // fact*cost, fact*sint, fact = sqrt(2)/2 (backprojection.fut:20)
+  int orig;                        // sinogram row (caller's angle index)
+  int flags;
+};
+
+struct Geom {
+  int A = 0, R = 0, N = 0, n_steep = 0, ncand = 0;
+  float rho0 = 0.f, drho = 1.f;
+  std::vector<Line> lines;   // host copy, steep lines then flat lines
+  Line* d_lines = nullptr;
+  // SIRT weights, computed on first use (SIRT.fut:22-25)
+  float* d_rowinv = nullptr;  // [A][R]
+  float* d_colinv = nullptr;  // [N][N]
+  int weights_exact = -1;     // SK_OPT_EXACT value the weights were computed with
+  int device = 0;
+  // Cached SIRT iteration graph and the arguments it was captured with.
+  hipGraphExec_t sirt_exec = nullptr;
+  struct {
+    const float* img = nullptr;
+    const float* b = nullptr;
+    const void* ws = nullptr;
+    int batch = 0, exact = -1, iters = 0;
+  } sirt_key;
+};
+
+struct Ctx {
+  int device = 0;
+  std::string err;
+  int exact = 0;
+  int fp_variant = 0;
+  int bp_variant = 0;
+  // growable scratch (stream-ordered use; grown after a stream sync)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  hipStream_t capture_stream = nullptr;  // private stream used for hipGraph capture
+};
+
+// Error helpers: record a message in the context and return the code.
+int fail(Ctx* c, int code, const char* fmt, ...);
+int hip_fail(Ctx* c, hipError_t e, const char* what);
+
+#define SK_HIP(ctx, expr)                                   \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return ::sk::hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+// Scratch memory of at least `bytes`, ordered after `stream`.
+int scratch(Ctx* c, size_t bytes, hipStream_t stream, void** out);
+
+// Kernel launchers (sk_proj.hip).  batch >= 1; strides are per-batch element counts.
+enum FpMode { FP_PLAIN = 0, FP_ONES_INV = 1, FP_SIRT = 2 };
+enum BpMode { BP_PLAIN = 0, BP_ONES_INV = 1, BP_SIRT = 2 };
+enum BpWeights { W_REF = 0, W_ADJ = 1 };
+
+struct FpArgs {
+  const float* img;    // [batch][N][N]  (unused in FP_ONES_INV)
+  const float* imgT;   // [batch][N][N] transposed copy (flat lines)
+  float* out;          // [batch][A][R]
+  const float* b;      // FP_SIRT: measured sinogram [batch][A][R]
+  const float* rowinv; // FP_SIRT: [A][R]
+  int batch;
+};
+struct BpArgs {
+  const float* sino;   // [batch][A][R]   (unused in BP_ONES_INV)
+  float* out;          // [batch][N][N]   (BP_SIRT: the image, updated in place)
+  float* outT;         // BP_SIRT: transposed copy of the updated image
+  const float* colinv; // BP_SIRT: [N][N]
+  int batch;
+};
+
+int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipStream_t s);
+int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s);
+int launch_bp(Ctx* c, const Geom* g, BpMode mode, BpWeights w, const BpArgs& a, hipStream_t s);
+
+}  // namespace sk
+
+struct sk_ctx {
+  sk::Ctx c;
+};
+struct sk_geom {
+  sk::Geom g;
+};

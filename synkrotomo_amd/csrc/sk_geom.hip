@@ -1,0 +1,212 @@
+// sk_geom.hip -- context, errors, scratch memory and the per-angle line table.
+//
+// The line table replaces sirtlib.preprocess (futhark/sirtlib.fut:28-44), which the
+// reference runs as a map + partition + scatter on the device at every call.  Here it is
+// built once per geometry on the host (A entries of 32 bytes) and uploaded; the kernels
+// index the sinogram through Line::orig, so neither the sinogram partition
+// (fix_projections, sirtlib.fut:16-21) nor the output scatter (postprocess_fp,
+// sirtlib.fut:24-25) exists.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "sk_common.h"
+
+namespace sk {
+
+int fail(Ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what) {
+  int code = (e == hipErrorOutOfMemory) ? SK_ENOMEM : SK_EHIP;
+  return fail(c, code, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+int scratch(Ctx* c, size_t bytes, hipStream_t stream, void** out) {
+  if (bytes > c->scratch_bytes) {
+    // Previously enqueued work may still use the old buffer.
+    SK_HIP(c, hipStreamSynchronize(stream));
+    if (c->scratch) SK_HIP(c, hipFree(c->scratch));
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    size_t want = bytes + (bytes >> 3);
+    SK_HIP(c, hipMalloc(&c->scratch, want));
+    c->scratch_bytes = want;
+  }
+  *out = c->scratch;
+  return SK_OK;
+}
+
+// Host line table.  Same operations and rounding points as sirtlib.fut:29-41:
+// cos/sin of the f32 angle, l = sqrt(1 + (a/b)**2) in f32, flat <=> |sin| >= |cos|,
+// stable partition steep-then-flat.
+static int build_lines(const float* ang, int A, std::vector<Line>& out) {
+  const float fact = sqrtf(2.0f) / 2.0f;
+  std::vector<Line> steep, flat;
+  steep.reserve(A);
+  flat.reserve(A);
+  for (int i = 0; i < A; i++) {
+    const float c = cosf(ang[i]);
+    const float s = sinf(ang[i]);
+    const float lsteep = sqrtf(1.0f + powf(s / c, 2.0f));
+    const float lflat = sqrtf(1.0f + powf(c / s, 2.0f));
+    const bool is_flat = fabsf(s) >= fabsf(c);
+    Line L{};
+    if (is_flat) {
+      L.cost = -s;
+      L.sint = c;
+      L.lbase = lflat;
+    } else {
+      L.cost = c;
+      L.sint = -s;
+      L.lbase = lsteep;
+    }
+    L.slope = -L.sint / L.cost;
+    L.fc = fact * L.cost;
+    L.fs = fact * L.sint;
+    L.orig = i;
+    L.flags = (is_flat ? LINE_FLAT : 0) | (L.slope >= 0.0f ? LINE_SLOPE_POS : 0) |
+              (fabsf(L.slope) < 1e-3f ? LINE_DEGEN : 0);
+    (is_flat ? flat : steep).push_back(L);
+  }
+  out = steep;
+  out.insert(out.end(), flat.begin(), flat.end());
+  return (int)steep.size();
+}
+
+}  // namespace sk
+
+using namespace sk;
+
+extern "C" {
+
+int sk_create(int device, sk_ctx** out) {
+  if (!out) return SK_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SK_EHIP;
+  if (device < 0 || device >= n) return SK_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return SK_EHIP;
+  sk_ctx* c = new sk_ctx();
+  c->c.device = device;
+  if (hipStreamCreateWithFlags(&c->c.capture_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SK_EHIP;
+  }
+  *out = c;
+  return SK_OK;
+}
+
+void sk_destroy(sk_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.scratch) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(ctx->c.scratch);
+  }
+  if (ctx->c.capture_stream) (void)hipStreamDestroy(ctx->c.capture_stream);
+  delete ctx;
+}
+
+const char* sk_last_error(const sk_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int sk_set_option(sk_ctx* ctx, int option, int value) {
+  if (!ctx) return SK_EINVAL;
+  switch (option) {
+    case SK_OPT_EXACT: ctx->c.exact = value ? 1 : 0; return SK_OK;
+    case SK_OPT_FP_VARIANT: ctx->c.fp_variant = value; return SK_OK;
+    case SK_OPT_BP_VARIANT: ctx->c.bp_variant = value; return SK_OK;
+    default: return fail(&ctx->c, SK_EINVAL, "unknown option %d", option);
+  }
+}
+
+int sk_get_option(const sk_ctx* ctx, int option, int* value) {
+  if (!ctx || !value) return SK_EINVAL;
+  switch (option) {
+    case SK_OPT_EXACT: *value = ctx->c.exact; return SK_OK;
+    case SK_OPT_FP_VARIANT: *value = ctx->c.fp_variant; return SK_OK;
+    case SK_OPT_BP_VARIANT: *value = ctx->c.bp_variant; return SK_OK;
+    default: return SK_EINVAL;
+  }
+}
+
+// Host-only: the line table as 8 values per line (cost, sint, lbase, slope, fc, fs,
+// orig, flags), steep lines first.  Returns the number of steep lines, or -1.
+int sk_lines_compute(const float* angles_host, int A, float* table_out) {
+  if (!angles_host || A < 1 || !table_out) return -1;
+  std::vector<Line> lines;
+  int ns = build_lines(angles_host, A, lines);
+  for (int j = 0; j < A; j++) {
+    const Line& L = lines[j];
+    float* t = table_out + 8 * j;
+    t[0] = L.cost; t[1] = L.sint; t[2] = L.lbase; t[3] = L.slope;
+    t[4] = L.fc; t[5] = L.fs; t[6] = (float)L.orig; t[7] = (float)L.flags;
+  }
+  return ns;
+}
+
+int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, float drho, int R,
+                   int N, sk_geom** out) {
+  if (!ctx || !out) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  *out = nullptr;
+  if (!angles_host) return fail(c, SK_EINVAL, "angles is NULL");
+  if (A < 1 || A > 65535) return fail(c, SK_EINVAL, "angle count must be in [1, 65535] (got %d)", A);
+  if (R < 1) return fail(c, SK_EINVAL, "numrhos must be positive (got %d)", R);
+  if (N < 2 || (N & 1)) return fail(c, SK_EINVAL, "image size must be even and >= 2 (got %d)", N);
+  if (N > 16384) return fail(c, SK_EINVAL, "image size %d exceeds 16384", N);
+  if (!(drho > 0.0f) || !std::isfinite(drho)) return fail(c, SK_EINVAL, "deltarho must be > 0");
+  if (!std::isfinite(rho0) || fabsf(rho0) > 1e6f)
+    return fail(c, SK_EINVAL, "rhozero must be finite and |rhozero| <= 1e6");
+  if ((double)R * drho > 4e6) return fail(c, SK_EINVAL, "detector extent numrhos*deltarho too large");
+  for (int i = 0; i < A; i++)
+    if (!std::isfinite(angles_host[i])) return fail(c, SK_EINVAL, "angle %d is not finite", i);
+  const int ncand = (int)ceilf(sqrtf(2.0f) / drho);  // backprojection.fut:87
+  if (ncand > 16) return fail(c, SK_EINVAL, "deltarho %g too small (more than 16 rays per pixel)", drho);
+  SK_HIP(c, hipSetDevice(c->device));
+  sk_geom* g = new sk_geom();
+  Geom& G = g->g;
+  G.A = A; G.R = R; G.N = N; G.rho0 = rho0; G.drho = drho; G.ncand = ncand;
+  G.device = c->device;
+  G.n_steep = build_lines(angles_host, A, G.lines);
+  hipError_t e = hipMalloc(&G.d_lines, sizeof(Line) * (size_t)A);
+  if (e == hipSuccess)
+    e = hipMemcpy(G.d_lines, G.lines.data(), sizeof(Line) * (size_t)A, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (G.d_lines) (void)hipFree(G.d_lines);
+    delete g;
+    return hip_fail(c, e, "geometry upload");
+  }
+  *out = g;
+  return SK_OK;
+}
+
+void sk_geom_destroy(sk_geom* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->g.device);
+  (void)hipDeviceSynchronize();
+  if (g->g.sirt_exec) (void)hipGraphExecDestroy(g->g.sirt_exec);
+  if (g->g.d_lines) (void)hipFree(g->g.d_lines);
+  if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
+  if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);
+  delete g;
+}
+
+int sk_geom_query(const sk_geom* g, int* A, int* R, int* N, int* n_steep, int* ncand) {
+  if (!g) return SK_EINVAL;
+  if (A) *A = g->g.A;
+  if (R) *R = g->g.R;
+  if (N) *N = g->g.N;
+  if (n_steep) *n_steep = g->g.n_steep;
+  if (ncand) *ncand = g->g.ncand;
+  return SK_OK;
+}
+
+}  // extern "C"
