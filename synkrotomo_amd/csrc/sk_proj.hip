@@ -29,17 +29,16 @@ constexpr int BP_LB = 32;   // lines per staged block
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Conservative frame-row range [i0, i1) outside which both candidate pixels of a ray with
-// this slope and intercept are out of bounds (|x| > h + 2 with a 2-row margin).
+// this slope and intercept are out of bounds (|x| > h + 2 with a 2-row margin).  Both
+// ends move monotonically (in the same direction) with the intercept, so the hull of the
+// ranges of a wave's first and last rays covers every ray between them.  Near-zero slopes
+// (theta ~ 0, pi/2) take the full range: there a ray is either in for every row or in for
+// none, and an "empty" range would break that monotonicity.
 __device__ __forceinline__ void row_clip(float slope, float icpt, int h, int& i0, int& i1) {
   const float lo = -(float)h - 2.0f, hi = (float)h + 2.0f;
   if (fabsf(slope) < 1e-6f) {
-    if (icpt < lo || icpt > hi) {
-      i0 = 0;
-      i1 = 0;
-    } else {
-      i0 = -h;
-      i1 = h;
-    }
+    i0 = -h;
+    i1 = h;
     return;
   }
   const float ta = (lo - icpt) / slope, tb = (hi - icpt) / slope;
