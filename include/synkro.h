@@ -114,6 +114,30 @@ int sk_sirt_weights(sk_ctx* ctx, sk_geom* g, float* d_rowinv, float* d_colinv, v
 int sk_sirt_step(sk_ctx* ctx, sk_geom* g, int batch, float* d_img, const float* d_sino,
                  const float* d_rowinv, const float* d_colinv, float* d_q, void* stream);
 
+/* ---------------------------------------------------------------- sparse path */
+/* forwardprojection_sparse.main / backprojection_sparse.main
+ * (futhark/forwardprojection_sparse.fut:26-40, futhark/backprojection_sparse.fut:26-37):
+ * same signatures as the dense mains, run on an explicit CSR matrix built on the GPU.
+ *   SK_MAT_FP  A: rows = rays (angle*R + bin), cols = pixels; entries in row-step order with
+ *              the forward projector's lengths (bitwise equal to the FP32 restatement).
+ *   SK_MAT_BP  B: rows = pixels, cols = sinogram entries; the backprojector's non-zero
+ *              intersect_fact lengths, lines in steep-then-flat order.
+ * 64-bit row pointers, 32-bit columns, f32 values, all owned by the matrix.  The call
+ * synchronizes `stream` once (to size the column/value arrays). */
+int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stream);
+/* nnz of SK_MAT_FP / SK_MAT_BP without building the matrix (synchronizes `stream`). */
+int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* stream);
+int sk_mat_info(const sk_mat* m, int64_t* rows, int64_t* cols, int64_t* nnz);
+/* Device pointers of the CSR arrays (valid until sk_mat_destroy). */
+int sk_mat_arrays(const sk_mat* m, const int64_t** d_rowptr, const int32_t** d_cols,
+                  const float** d_vals);
+/* Copy the CSR arrays into caller-owned device buffers (rows+1, nnz, nnz entries). */
+int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols, float* d_vals,
+                void* stream);
+/* d_out[batch][rows] = M . d_in[batch][cols] */
+int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);
+void sk_mat_destroy(sk_mat* m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,13 @@ SIGNATURES = {
     "sk_sirt": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "sk_sirt_weights": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sk_sirt_step": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_sparse_build": (_i, [_vp, _vp, _i, ctypes.POINTER(_vp), _vp]),
+    "sk_sparse_count": (_i, [_vp, _vp, _i, _i64p, _vp]),
+    "sk_mat_info": (_i, [_vp, _i64p, _i64p, _i64p]),
+    "sk_mat_arrays": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "sk_mat_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_spmv": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_mat_destroy": (None, [_vp]),
 }
 
 

@@ -1,0 +1,459 @@
+// sk_sparse.hip -- explicit system matrices built on the GPU, and their SpMV.
+//
+// The reference's *_sparse entry points (futhark/forwardprojection_sparse.fut:26-40,
+// futhark/backprojection_sparse.fut:26-37) have the dense bodies; here they run on
+// explicit matrices:
+//   A (SK_MAT_FP): rows = rays in angle-major order (angle*R + bin), columns = pixels of the
+//                  caller's image.  Entries per ray in row-step order: pixel Xb when in
+//                  bounds, then Xt when it differs and is in bounds, with the lengths lb, lt
+//                  of forwardprojection.fut:37-54 (IEEE division -> bitwise those of fp).
+//   B (SK_MAT_BP): rows = pixels, columns = sinogram entries (angle*R + bin).  Entries per
+//                  pixel: steep lines then flat lines, candidates in order, those inside
+//                  [0, R) with non-zero intersect_fact length (backprojection.fut:44-78).
+//                  B is the reference's backprojector, which is A^T except for edge rays on
+//                  lines whose slope rounds to zero.
+// CSR, 64-bit row pointers (nnz exceeds 2^31 at 2048^2), 32-bit columns, f32 values.
+//
+// Build = count pass -> 64-bit exclusive scan -> fill pass.  Both passes are
+// wavefront-cooperative: one wave per row, lanes over the row's steps (A: 64 image rows at
+// a time; B: 64 lines at a time); ballots and mbcnt place each lane's entries, so the
+// entries come out in the sequential order above and the writes are contiguous.
+#include "sk_common.h"
+
+struct sk_mat {
+  int kind = 0;
+  int64_t rows = 0, cols = 0, nnz = 0;
+  int64_t* rowptr = nullptr;  // [rows + 1]
+  int32_t* col = nullptr;     // [nnz]
+  float* val = nullptr;       // [nnz]
+  int device = 0;
+};
+
+namespace sk {
+
+constexpr int SP_BLOCK = 256;  // 4 waves, one row each
+
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ int prefix_of(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// ------------------------------------------------------------------ A: one wave per ray
+template <bool FILL>
+__global__ __launch_bounds__(SP_BLOCK) void k_fp_rows(const Line* __restrict__ lines, int A, int R,
+                                                      int N, float rho0, float drho,
+                                                      int32_t* __restrict__ counts,
+                                                      const int64_t* __restrict__ rowptr,
+                                                      int32_t* __restrict__ cols,
+                                                      float* __restrict__ vals) {
+  const int lane = lane_id();
+  const long ray = (long)blockIdx.x * (SP_BLOCK / 64) + (threadIdx.x >> 6);
+  if (ray >= (long)A * R) return;  // whole wave
+  const int j = (int)(ray / R), r = (int)(ray - (long)j * R);
+  const Line L = lines[j];
+  const int h = N >> 1;
+  const bool flat = L.flags & LINE_FLAT;
+  const float slope = L.slope, lbase = L.lbase;
+  const float icpt = (rho0 + (float)r * drho) / L.cost;  // forwardprojection.fut:33-34
+  const long row = (long)L.orig * R + r;
+  long pos = FILL ? rowptr[row] : 0;
+  int total = 0;
+  for (int i0 = -h; i0 < h; i0 += 64) {
+    const int i = i0 + lane;
+    const bool live = i < h;
+    const float xbot = (float)i * slope + icpt;
+    const float xtop = xbot + slope;
+    const int Xb = (int)floorf(xbot), Xt = (int)floorf(xtop);
+    const bool single = Xb == Xt;
+    const bool inb = live && Xb >= -h && Xb < h;
+    const bool intop = live && !single && Xt >= -h && Xt < h;
+    const unsigned long long b1 = __ballot(inb || intop);
+    const unsigned long long b2 = __ballot(inb && intop);
+    if (FILL) {
+      const float Xmax = (float)(Xb > Xt ? Xb : Xt);
+      const float xdiff = xtop - xbot;
+      const float lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
+      const float lt = ((xtop - Xmax) / xdiff) * lbase;
+      long p = pos + prefix_of(b1) + prefix_of(b2);
+      if (inb) {
+        cols[p] = flat ? (Xb + h) * N + (i + h) : (i + h) * N + (Xb + h);
+        vals[p] = lb;
+        p++;
+      }
+      if (intop) {
+        cols[p] = flat ? (Xt + h) * N + (i + h) : (i + h) * N + (Xt + h);
+        vals[p] = lt;
+      }
+    }
+    const int n = __popcll(b1) + __popcll(b2);
+    pos += n;
+    total += n;
+  }
+  if (!FILL && lane == 0) counts[row] = total;
+}
+
+// ------------------------------------------------------------------ B: one wave per pixel
+template <bool FILL>
+__global__ __launch_bounds__(SP_BLOCK) void k_bp_rows(const Line* __restrict__ lines, int A, int R,
+                                                      int N, float rho0, float drho, int ncand,
+                                                      int32_t* __restrict__ counts,
+                                                      const int64_t* __restrict__ rowptr,
+                                                      int32_t* __restrict__ cols,
+                                                      float* __restrict__ vals) {
+  const int lane = lane_id();
+  const long pix = (long)blockIdx.x * (SP_BLOCK / 64) + (threadIdx.x >> 6);
+  if (pix >= (long)N * N) return;
+  const int Y = (int)(pix / N), X = (int)(pix - (long)Y * N);
+  const int h = N >> 1;
+  long pos = FILL ? rowptr[pix] : 0;
+  int total = 0;
+  for (int j0 = 0; j0 < A; j0 += 64) {
+    const int j = j0 + lane;
+    const bool live = j < A;
+    Line L = lines[live ? j : 0];
+    const bool fl = L.flags & LINE_FLAT;
+    const float xmin = (float)((fl ? Y : X) - h);
+    const float ymin = (float)((fl ? X : Y) - h);
+    // backprojection.fut:50-54
+    const float px = (xmin + 0.5f) - L.fc;
+    const float py = (ymin + 0.5f) - L.fs;
+    const float rho = L.cost * px + L.sint * py;
+    const float s = ceilf((rho - rho0) / drho);
+    const float xbase = ymin * L.slope;
+    int before = 0;  // entries of lower lanes
+    int mine = 0;    // entries of this lane so far
+    unsigned long long masks[16];
+    float lens[16];
+    int sps[16];
+    for (int k = 0; k < ncand; ++k) {
+      // backprojection.fut:56-64
+      const float sp = s + (float)k;
+      const float rr = sp * drho + rho0;
+      const float xbot = xbase + (rr / L.cost);
+      const float xtop = xbot + L.slope;
+      const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
+      const float lo = fmaxf(maxx - xmin, 0.0f);
+      const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
+      const float diff = maxx - minx;
+      const float m = fminf(lo, hi);
+      const float p = (m == 0.0f) ? 0.0f : m / diff;
+      const float l = fminf(p, 1.0f) * L.lbase;
+      const int si = (int)sp;
+      const bool ok = live && si >= 0 && si < R && l != 0.0f;
+      masks[k] = __ballot(ok);
+      lens[k] = l;
+      sps[k] = ok ? si : -1;
+      before += prefix_of(masks[k]);
+    }
+    if (FILL) {
+      long p = pos + before;
+      for (int k = 0; k < ncand; ++k) {
+        if (sps[k] >= 0) {
+          cols[p + mine] = L.orig * R + sps[k];
+          vals[p + mine] = lens[k];
+          mine++;
+        }
+      }
+    }
+    int n = 0;
+    for (int k = 0; k < ncand; ++k) n += __popcll(masks[k]);
+    pos += n;
+    total += n;
+  }
+  if (!FILL && lane == 0) counts[pix] = total;
+}
+
+// ------------------------------------------------------------------ 64-bit exclusive scan
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
+
+__device__ __forceinline__ long block_excl_scan(long v, long* sh, long* total) {
+  // Hillis-Steele over 256 threads in LDS (scan of per-thread sums).
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    long add = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  long incl = sh[t];
+  *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t* __restrict__ in, long n,
+                                                    int64_t* __restrict__ out,
+                                                    int64_t* __restrict__ tile_sums) {
+  __shared__ long sh[256];
+  const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  long v[SCAN_ITEMS];
+  long s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  long total;
+  long run = block_excl_scan(s, sh, &total);
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block scans the tile sums (any count) in place to exclusive offsets; writes the
+// grand total to *grand.
+__global__ __launch_bounds__(256) void k_scan_sums(int64_t* __restrict__ sums, long ntiles,
+                                                   int64_t* __restrict__ grand) {
+  __shared__ long sh[256];
+  long carry = 0;
+  for (long b0 = 0; b0 < ntiles; b0 += 256) {
+    const long i = b0 + threadIdx.x;
+    const long v = i < ntiles ? sums[i] : 0;
+    long total;
+    const long ex = block_excl_scan(v, sh, &total);
+    if (i < ntiles) sums[i] = carry + ex;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int64_t* __restrict__ out, long n,
+                                                  const int64_t* __restrict__ sums,
+                                                  const int64_t* __restrict__ grand) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] += sums[i / SCAN_TILE];
+  if (i == 0) out[n] = *grand;
+}
+
+// ------------------------------------------------------------------ SpMV: one wave per row
+template <int UNROLL>
+__global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ col,
+                                              const float* __restrict__ val, long rows, long ncols,
+                                              const float* __restrict__ x, float* __restrict__ y) {
+  const int lane = lane_id();
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const long beg = rowptr[row], end = rowptr[row + 1];
+  float acc = 0.0f;
+  long e = beg + lane;
+  for (; e + 64 * (UNROLL - 1) < end; e += 64 * UNROLL) {
+    int c[UNROLL];
+    float v[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      c[u] = __builtin_nontemporal_load(col + e + 64 * u);
+      v[u] = __builtin_nontemporal_load(val + e + 64 * u);
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], xb[c[u]], acc);
+  }
+  for (; e < end; e += 64) acc = fmaf(__builtin_nontemporal_load(val + e), xb[__builtin_nontemporal_load(col + e)], acc);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
+static int launched(Ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, what);
+  return SK_OK;
+}
+
+// counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
+static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr, int64_t* nnz,
+                       hipStream_t s) {
+  const long ntiles = (rows + SCAN_TILE - 1) / SCAN_TILE;
+  void* ws = nullptr;
+  int rc = scratch(c, sizeof(int64_t) * (size_t)(ntiles + 1), s, &ws);
+  if (rc) return rc;
+  int64_t* sums = (int64_t*)ws;
+  int64_t* grand = sums + ntiles;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, s, counts, rows, rowptr, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, sums, ntiles, grand);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, rowptr, rows,
+                     sums, grand);
+  rc = launched(c, "scan kernels");
+  if (rc) return rc;
+  SK_HIP(c, hipMemcpyAsync(nnz, rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SK_HIP(c, hipStreamSynchronize(s));
+  return SK_OK;
+}
+
+static void launch_rows(const Geom& G, int kind, bool fill, int32_t* counts, const int64_t* rowptr,
+                        int32_t* cols, float* vals, hipStream_t s) {
+  const long rows = kind == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
+  const unsigned blocks = (unsigned)((rows + 3) / 4);
+  if (kind == SK_MAT_FP) {
+    if (fill)
+      hipLaunchKernelGGL(k_fp_rows<true>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
+                         G.rho0, G.drho, counts, rowptr, cols, vals);
+    else
+      hipLaunchKernelGGL(k_fp_rows<false>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
+                         G.rho0, G.drho, counts, rowptr, cols, vals);
+  } else {
+    if (fill)
+      hipLaunchKernelGGL(k_bp_rows<true>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
+                         G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
+    else
+      hipLaunchKernelGGL(k_bp_rows<false>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
+                         G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
+  }
+}
+
+}  // namespace sk
+
+using namespace sk;
+
+static void mat_free(sk_mat* m) {
+  if (!m) return;
+  if (m->rowptr) (void)hipFree(m->rowptr);
+  if (m->col) (void)hipFree(m->col);
+  if (m->val) (void)hipFree(m->val);
+  delete m;
+}
+
+extern "C" {
+
+int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stream) {
+  if (!ctx || !out) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  *out = nullptr;
+  if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
+  if (which != SK_MAT_FP && which != SK_MAT_BP) return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
+  const Geom& G = g->g;
+  if ((long)G.A * G.R >= (1L << 31) || (long)G.N * G.N >= (1L << 31))
+    return fail(c, SK_EINVAL, "matrix dimensions exceed 32-bit column indices");
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  sk_mat* m = new sk_mat();
+  m->kind = which;
+  m->device = c->device;
+  m->rows = which == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
+  m->cols = which == SK_MAT_FP ? (long)G.N * G.N : (long)G.A * G.R;
+  hipError_t e = hipMalloc(&m->rowptr, sizeof(int64_t) * (size_t)(m->rows + 1));
+  if (e != hipSuccess) {
+    mat_free(m);
+    return hip_fail(c, e, "row pointer allocation");
+  }
+  int32_t* counts = nullptr;
+  e = hipMalloc(&counts, sizeof(int32_t) * (size_t)m->rows);
+  if (e != hipSuccess) {
+    mat_free(m);
+    return hip_fail(c, e, "count allocation");
+  }
+  launch_rows(G, which, false, counts, nullptr, nullptr, nullptr, s);
+  int rc = launched(c, "sparse count kernel");
+  int64_t nnz = 0;
+  if (!rc) rc = scan_counts(c, counts, m->rows, m->rowptr, &nnz, s);
+  (void)hipFree(counts);
+  if (rc) {
+    mat_free(m);
+    return rc;
+  }
+  m->nnz = nnz;
+  if (nnz > 0) {
+    e = hipMalloc(&m->col, sizeof(int32_t) * (size_t)nnz);
+    if (e == hipSuccess) e = hipMalloc(&m->val, sizeof(float) * (size_t)nnz);
+    if (e != hipSuccess) {
+      mat_free(m);
+      return hip_fail(c, e, "matrix allocation");
+    }
+    launch_rows(G, which, true, nullptr, m->rowptr, m->col, m->val, s);
+    rc = launched(c, "sparse fill kernel");
+    if (rc) {
+      mat_free(m);
+      return rc;
+    }
+  }
+  *out = m;
+  return SK_OK;
+}
+
+int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* stream) {
+  if (!ctx || !nnz_out) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
+  if (which != SK_MAT_FP && which != SK_MAT_BP) return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
+  const Geom& G = g->g;
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const long rows = which == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
+  int32_t* counts = nullptr;
+  int64_t* rowptr = nullptr;
+  SK_HIP(c, hipMalloc(&counts, sizeof(int32_t) * (size_t)rows));
+  hipError_t e = hipMalloc(&rowptr, sizeof(int64_t) * (size_t)(rows + 1));
+  if (e != hipSuccess) {
+    (void)hipFree(counts);
+    return hip_fail(c, e, "row pointer allocation");
+  }
+  launch_rows(G, which, false, counts, nullptr, nullptr, nullptr, s);
+  int rc = launched(c, "sparse count kernel");
+  if (!rc) rc = scan_counts(c, counts, rows, rowptr, nnz_out, s);
+  (void)hipFree(counts);
+  (void)hipFree(rowptr);
+  return rc;
+}
+
+int sk_mat_info(const sk_mat* m, int64_t* rows, int64_t* cols, int64_t* nnz) {
+  if (!m) return SK_EINVAL;
+  if (rows) *rows = m->rows;
+  if (cols) *cols = m->cols;
+  if (nnz) *nnz = m->nnz;
+  return SK_OK;
+}
+
+int sk_mat_arrays(const sk_mat* m, const int64_t** d_rowptr, const int32_t** d_cols,
+                  const float** d_vals) {
+  if (!m) return SK_EINVAL;
+  if (d_rowptr) *d_rowptr = m->rowptr;
+  if (d_cols) *d_cols = m->col;
+  if (d_vals) *d_vals = m->val;
+  return SK_OK;
+}
+
+int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols, float* d_vals,
+                void* stream) {
+  if (!ctx) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  if (!m) return fail(c, SK_EINVAL, "matrix is NULL");
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  if (d_rowptr)
+    SK_HIP(c, hipMemcpyAsync(d_rowptr, m->rowptr, sizeof(int64_t) * (size_t)(m->rows + 1),
+                             hipMemcpyDeviceToDevice, s));
+  if (d_cols && m->nnz)
+    SK_HIP(c, hipMemcpyAsync(d_cols, m->col, sizeof(int32_t) * (size_t)m->nnz, hipMemcpyDeviceToDevice, s));
+  if (d_vals && m->nnz)
+    SK_HIP(c, hipMemcpyAsync(d_vals, m->val, sizeof(float) * (size_t)m->nnz, hipMemcpyDeviceToDevice, s));
+  return SK_OK;
+}
+
+int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream) {
+  if (!ctx) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  if (!m) return fail(c, SK_EINVAL, "matrix is NULL");
+  if (!d_in || !d_out) return fail(c, SK_EINVAL, "device pointer is NULL");
+  if (batch < 1 || batch > 65535) return fail(c, SK_EINVAL, "batch must be in [1, 65535]");
+  if (m->device != c->device) return fail(c, SK_EINVAL, "matrix belongs to another device");
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)((m->rows + 3) / 4), batch);
+  hipLaunchKernelGGL(k_spmv<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val, (long)m->rows,
+                     (long)m->cols, d_in, d_out);
+  return launched(c, "SpMV kernel");
+}
+
+void sk_mat_destroy(sk_mat* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  (void)hipDeviceSynchronize();
+  mat_free(m);
+}
+
+}  // extern "C"

@@ -7,6 +7,8 @@
 Modules and classes keep the reference's names and ``main`` argument orders
 (README.md:21-25, Makefile:1-4); each call runs hand-written HIP through libsynkro.so.
 """
-from . import SIRT, SIRT3D, backprojection, forwardprojection  # noqa: F401
+from . import (SIRT, SIRT3D, backprojection, backprojection_sparse,  # noqa: F401
+               forwardprojection, forwardprojection_sparse)
 
-__all__ = ["SIRT", "SIRT3D", "backprojection", "forwardprojection"]
+__all__ = ["SIRT", "SIRT3D", "backprojection", "backprojection_sparse", "forwardprojection",
+           "forwardprojection_sparse"]

@@ -111,6 +111,9 @@ class Geometry:
         self.A, self.R, self.N, self.n_steep, self.ncand = A.value, R.value, N.value, ns.value, nc.value
 
     def close(self) -> None:
+        for m in getattr(self, "_mats", {}).values():
+            m.close()
+        self._mats = {}
         if self.handle:
             nat.lib().sk_geom_destroy(self.handle)
             self.handle = None
@@ -120,6 +123,27 @@ class Geometry:
             self.close()
         except Exception:
             pass
+
+    def sparse(self, kind: str) -> "Matrix":
+        """The explicit matrix of this geometry ("fp": A, "bp": B), built once and cached."""
+        mats = self.__dict__.setdefault("_mats", {})
+        m = mats.get(kind)
+        if m is None:
+            m = mats[kind] = Matrix(self, kind)
+        return m
+
+    def drop_sparse(self) -> None:
+        for m in getattr(self, "_mats", {}).values():
+            m.close()
+        self._mats = {}
+
+    def sparse_nnz(self, kind: str) -> int:
+        """nnz of the explicit matrix without building it."""
+        which = {"fp": nat.SK_MAT_FP, "bp": nat.SK_MAT_BP}[kind]
+        v = ctypes.c_int64()
+        nat.check(nat.lib().sk_sparse_count(self.ctx.handle, self.handle, which, ctypes.byref(v),
+                                            _stream(self.ctx.device)), self.ctx.handle, "sparse count")
+        return v.value
 
     # helpers ---------------------------------------------------------------
     @property
@@ -218,6 +242,59 @@ class Geometry:
                                          _ptr(rowinv), _ptr(colinv), _ptr(q), _stream(self.ctx.device)),
                   self.ctx.handle, "SIRT step")
         return img
+
+
+class Matrix:
+    """An explicit system matrix in CSR form, built on the GPU (``sk_sparse_build``).
+
+    kind "fp": A, rows = rays (angle*R + bin), columns = pixels.
+    kind "bp": B, rows = pixels, columns = sinogram entries (the reference backprojector).
+    """
+
+    def __init__(self, geom: "Geometry", kind: str):
+        which = {"fp": nat.SK_MAT_FP, "bp": nat.SK_MAT_BP}[kind]
+        self.geom = geom
+        self.kind = kind
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().sk_sparse_build(geom.ctx.handle, geom.handle, which, ctypes.byref(h),
+                                            _stream(geom.ctx.device)), geom.ctx.handle, "sparse build")
+        self.handle = h
+        rows, cols, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        nat.lib().sk_mat_info(h, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz))
+        self.rows, self.cols, self.nnz = rows.value, cols.value, nnz.value
+
+    def close(self) -> None:
+        if self.handle:
+            nat.lib().sk_mat_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def csr(self):
+        """(rowptr int64 [rows+1], cols int32 [nnz], vals float32 [nnz]) as new device tensors."""
+        dev = self.geom.device
+        rp = torch.empty(self.rows + 1, dtype=torch.int64, device=dev)
+        c = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        v = torch.empty(max(self.nnz, 1), dtype=torch.float32, device=dev)
+        nat.check(nat.lib().sk_mat_copy(self.geom.ctx.handle, self.handle, _ptr(rp), _ptr(c), _ptr(v),
+                                        _stream(self.geom.ctx.device)), self.geom.ctx.handle, "matrix copy")
+        return rp, c[: self.nnz], v[: self.nnz]
+
+    def spmv(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        g = self.geom
+        batch = g._check(x, self.cols, "input")
+        if out is None:
+            out = torch.empty((batch, self.rows) if batch > 1 else (self.rows,), dtype=torch.float32,
+                              device=g.device)
+        elif g._check(out, self.rows, "out") != batch:
+            raise ValueError("out batch mismatch")
+        nat.check(nat.lib().sk_spmv(g.ctx.handle, self.handle, batch, _ptr(x), _ptr(out),
+                                    _stream(g.ctx.device)), g.ctx.handle, "SpMV")
+        return out
 
 
 def uniform_angles(A: int) -> np.ndarray:
