@@ -57,6 +57,37 @@ def cpu_baseline(angles, phantom, sino) -> dict:
             "bp_gups": round(N * N * ang.size / (t2 - t1) / 1e9, 3)}
 
 
+def sparse_leg(geo, x, sino, K: int) -> dict:
+    """Explicit CSR A and B at the same geometry, built and timed one at a time (each is
+    ~87 GB at 2048^2).  SpMV bytes = 8*nnz + 8*(rows+1) + 4*(rows+cols)."""
+    import torch
+    res = {}
+    for kind, inp in (("fp", x.reshape(-1)), ("bp", sino.reshape(-1))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = geo.sparse(kind)
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        out = torch.empty(m.rows, device="cuda")
+        m.spmv(inp, out=out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(K):
+            m.spmv(inp, out=out)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / K
+        nbytes = 8.0 * m.nnz + 8.0 * (m.rows + 1) + 4.0 * (m.rows + m.cols)
+        res[kind] = {"ms": round(ms, 3), "gups": round(float(N) * N * A / (ms * 1e-3) / 1e9, 2),
+                     "build_ms": round(build_ms, 1), "nnz": m.nnz,
+                     "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        geo.drop_sparse()
+        del out
+        torch.cuda.empty_cache()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +95,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--exact", action="store_true", help="IEEE division everywhere (bitwise mode)")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the explicit-matrix leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the projector shards by "
@@ -136,7 +168,10 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes()},
         "cpu_baseline": None,
+        "sparse": None,
     }
+    if not args.no_sparse:
+        out["sparse"] = sparse_leg(geo, x, sino, max(3, K // 2))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(angles, phantom.ravel(), sino_host.ravel())
     print(json.dumps(out), flush=True)
