@@ -96,6 +96,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--exact", action="store_true", help="IEEE division everywhere (bitwise mode)")
     ap.add_argument("--no-sparse", action="store_true", help="skip the explicit-matrix leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device copy first (reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the projector shards by "
@@ -109,6 +111,12 @@ def main() -> int:
 
     ctx = Context.default(0)
     ctx.exact = args.exact
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     angles = uniform_angles(A)
     phantom = ellipse_phantom(N, PHANTOM_ELLIPSES, seed=PHANTOM_SEED)
     geo = ctx.geometry(angles, RHO0, 1.0, R, N)

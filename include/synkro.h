@@ -97,6 +97,12 @@ int sk_bp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img,
 int sk_bp_adjoint(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img,
                   void* stream);
 
+/* B^T.v, the exact adjoint of sk_bp: the ray walk of sk_fp with the backprojector's
+ * lengths.  d_img [batch][N][N] -> d_sino [batch][A][R].  No reference counterpart; used by
+ * the backward of the differentiable SIRT step. */
+int sk_bp_transpose(sk_ctx* ctx, sk_geom* g, int batch, const float* d_img, float* d_sino,
+                    void* stream);
+
 /* ---------------------------------------------------------------- SIRT */
 /* SIRT.main (futhark/SIRT.fut:9-49): x <- x + C*B(R*(b - A x)) `iters` times, in place
  * on d_img [batch][N][N], with R = 1/rowsum(A), C = 1/colsum(B) (0 where the sum is 0).

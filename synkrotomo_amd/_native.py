@@ -40,6 +40,7 @@ SIGNATURES = {
     "sk_fp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_bp_transpose": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_sirt": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "sk_sirt_weights": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sk_sirt_step": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -97,6 +97,7 @@ struct BpArgs {
 
 int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipStream_t s);
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s);
+int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s);
 int launch_bp(Ctx* c, const Geom* g, BpMode mode, BpWeights w, const BpArgs& a, hipStream_t s);
 
 }  // namespace sk

@@ -68,9 +68,46 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
 }
 
 // ------------------------------------------------------------------------ forward projector
-template <bool EXACT, int MODE>
+// First candidate ray of a pixel (backprojection.fut:51-54).
+template <bool DR1>
+__device__ __forceinline__ float cand_s(const Line& L, float xmin, float ymin, float rho0,
+                                        float drho) {
+  const float px = (xmin + 0.5f) - L.fc;
+  const float py = (ymin + 0.5f) - L.fs;
+  const float rho = L.cost * px + L.sint * py;
+  float t = rho - rho0;
+  if constexpr (!DR1) t = t / drho;
+  return ceilf(t);
+}
+
+// Length B[pixel, ray] of the reference backprojector for the pixel in frame column X of
+// frame row y (= the ray's row): intersect_fact of the ray's [xbot, xtop] with [X, X+1),
+// if the ray is one of the pixel's candidates s..s+ncand-1 (backprojection.fut:50-62).
+template <bool EXACT, bool DR1>
+__device__ __forceinline__ float bp_entry(const Line& L, int X, float y, float xbot, float xtop,
+                                          int r, int ncand, float rho0, float drho) {
+  const float xmin = (float)X;
+  const float s = cand_s<DR1>(L, xmin, y, rho0, drho);
+  const int si = (int)s;
+  const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
+  const float lo = fmaxf(maxx - xmin, 0.0f);
+  const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
+  const float diff = maxx - minx;
+  const float m = fminf(lo, hi);
+  float p;
+  if constexpr (EXACT)
+    p = (m == 0.0f) ? 0.0f : m / diff;
+  else
+    p = (m == 0.0f) ? 0.0f : m * fast_rcp(diff);
+  const float l = fminf(p, 1.0f) * L.lbase;
+  return (r >= si && r < si + ncand) ? l : 0.0f;
+}
+
+// WEIGHTS == W_REF: A (the forward projector).  WEIGHTS == W_ADJ here means B^T: the same
+// ray walk with the backprojector's lengths (the exact adjoint of k_bp).
+template <bool EXACT, int MODE, int WEIGHTS = W_REF, bool DR1 = true>
 __global__ __launch_bounds__(FP_BLOCK) void k_fp(const Line* __restrict__ lines, int A, int R,
-                                                 int N, float rho0, float drho,
+                                                 int N, float rho0, float drho, int ncand,
                                                  const float* __restrict__ img,
                                                  const float* __restrict__ imgT,
                                                  float* __restrict__ out,
@@ -114,7 +151,10 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp(const Line* __restrict__ lines,
     const bool inb = active && (unsigned)(Xb + h) < (unsigned)N;
     const bool intop = active && !single && (unsigned)(Xt + h) < (unsigned)N;
     float lb, lt;
-    if constexpr (EXACT) {
+    if constexpr (WEIGHTS == W_ADJ) {
+      lb = bp_entry<EXACT, DR1>(L, Xb, (float)i, xbot, xtop, r, ncand, rho0, drho);
+      lt = bp_entry<EXACT, DR1>(L, Xt, (float)i, xbot, xtop, r, ncand, rho0, drho);
+    } else if constexpr (EXACT) {
       lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
       lt = ((xtop - Xmax) / xdiff) * lbase;
     } else {
@@ -146,18 +186,6 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp(const Line* __restrict__ lines,
 }
 
 // ------------------------------------------------------------------------ backprojector
-// First candidate ray of a pixel (backprojection.fut:51-54).
-template <bool DR1>
-__device__ __forceinline__ float cand_s(const Line& L, float xmin, float ymin, float rho0,
-                                        float drho) {
-  const float px = (xmin + 0.5f) - L.fc;
-  const float py = (ymin + 0.5f) - L.fs;
-  const float rho = L.cost * px + L.sint * py;
-  float t = rho - rho0;
-  if constexpr (!DR1) t = t / drho;
-  return ceilf(t);
-}
-
 // Length of one candidate ray in one pixel.
 //   W_REF: intersect_fact(max, min, xmin, xmin+1) * lbase (backprojection.fut:22-33,59-62)
 //   W_ADJ: the forward projector's length for this (ray, row) if its Xb or Xt is this
@@ -304,11 +332,22 @@ int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipS
   return launched(c, "transpose kernel");
 }
 
-template <bool EXACT, int MODE>
+template <bool EXACT, int MODE, int WEIGHTS = W_REF, bool DR1 = true>
 static void fp_go(const Geom* g, const FpArgs& a, hipStream_t s) {
   dim3 grid((g->R + FP_BLOCK - 1) / FP_BLOCK, g->A, a.batch);
-  hipLaunchKernelGGL((k_fp<EXACT, MODE>), grid, dim3(FP_BLOCK), 0, s, g->d_lines, g->A, g->R,
-                     g->N, g->rho0, g->drho, a.img, a.imgT, a.out, a.b, a.rowinv);
+  hipLaunchKernelGGL((k_fp<EXACT, MODE, WEIGHTS, DR1>), grid, dim3(FP_BLOCK), 0, s, g->d_lines,
+                     g->A, g->R, g->N, g->rho0, g->drho, g->ncand, a.img, a.imgT, a.out, a.b,
+                     a.rowinv);
+}
+
+int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s) {
+  const bool ex = c->exact != 0;
+  const bool dr1 = g->drho == 1.0f;
+  if (ex)
+    dr1 ? fp_go<true, FP_PLAIN, W_ADJ, true>(g, a, s) : fp_go<true, FP_PLAIN, W_ADJ, false>(g, a, s);
+  else
+    dr1 ? fp_go<false, FP_PLAIN, W_ADJ, true>(g, a, s) : fp_go<false, FP_PLAIN, W_ADJ, false>(g, a, s);
+  return launched(c, "backprojection-transpose kernel");
 }
 
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s) {
@@ -414,6 +453,26 @@ int sk_bp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img,
   if (rc) return rc;
   BpArgs a{d_sino, d_img, nullptr, nullptr, batch};
   return launch_bp(&ctx->c, &g->g, BP_PLAIN, W_REF, a, (hipStream_t)stream);
+}
+
+int sk_bp_transpose(sk_ctx* ctx, sk_geom* g, int batch, const float* d_img, float* d_sino,
+                    void* stream) {
+  int rc = check_common(ctx, g, batch, d_img, d_sino);
+  if (rc) return rc;
+  Ctx* c = &ctx->c;
+  const Geom& G = g->g;
+  hipStream_t s = (hipStream_t)stream;
+  const float* imgT = d_img;
+  if (G.n_steep < G.A) {
+    void* ws = nullptr;
+    rc = scratch(c, sizeof(float) * (size_t)batch * G.N * G.N, s, &ws);
+    if (rc) return rc;
+    rc = launch_transpose(c, d_img, (float*)ws, G.N, batch, s);
+    if (rc) return rc;
+    imgT = (const float*)ws;
+  }
+  FpArgs a{d_img, imgT, d_sino, nullptr, nullptr, batch};
+  return launch_fp_bpt(c, &G, a, s);
 }
 
 int sk_bp_adjoint(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img,

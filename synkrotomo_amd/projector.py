@@ -210,6 +210,17 @@ class Geometry:
                                           _stream(self.ctx.device)), self.ctx.handle, "adjoint")
         return out
 
+    def bp_transpose(self, img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """B^T.v, the exact adjoint of bp."""
+        batch = self._check(img, self.N * self.N, "image")
+        if out is None:
+            out = torch.empty(self._sino_shape(img), dtype=torch.float32, device=self.device)
+        elif self._check(out, self.A * self.R, "out") != batch:
+            raise ValueError("out batch mismatch")
+        nat.check(nat.lib().sk_bp_transpose(self.ctx.handle, self.handle, batch, _ptr(img), _ptr(out),
+                                            _stream(self.ctx.device)), self.ctx.handle, "bp transpose")
+        return out
+
     def sirt_(self, img: torch.Tensor, sino: torch.Tensor, iterations: int) -> torch.Tensor:
         """SIRT in place on img (SIRT.main; batch > 1 = independent slices)."""
         batch = self._check(img, self.N * self.N, "image")
