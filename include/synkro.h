@@ -120,6 +120,30 @@ int sk_sirt_weights(sk_ctx* ctx, sk_geom* g, float* d_rowinv, float* d_colinv, v
 int sk_sirt_step(sk_ctx* ctx, sk_geom* g, int batch, float* d_img, const float* d_sino,
                  const float* d_rowinv, const float* d_colinv, float* d_q, void* stream);
 
+/* ---------------------------------------------------------------- batched SIRT */
+/* A group of `batch` independent problems sharing one geometry: d_img [batch][N][N]
+ * (initial images, updated in place), d_sino [batch][A][R]. */
+typedef struct {
+  sk_geom* geom;
+  int batch;
+  float* d_img;
+  const float* d_sino;
+} sk_problem;
+
+/* No reference counterpart (generalizes SIRT3D.main, futhark/SIRT3D.fut:18-28, which maps
+ * SIRT over slices of one geometry): advance every problem of every group `iters` SIRT
+ * iterations.  Each iteration is two persistent launches -- one work queue over all
+ * problems' forward-projection items, one over all their backprojection tiles -- and the
+ * whole sequence runs as one hipGraph.  Synchronizes `stream` before returning. */
+int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream);
+
+/* No reference counterpart: time the kernel variants (fp: 1 or 2 rays per lane; bp: 16x16,
+ * 16x32, 16x64 pixel tiles) on this geometry and keep the fastest for it.  Variants compute
+ * identical results.  Synchronizes `stream`. */
+int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
+/* Apply previously tuned variants (-1 = the context default). */
+int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant);
+
 /* ---------------------------------------------------------------- sparse path */
 /* forwardprojection_sparse.main / backprojection_sparse.main
  * (futhark/forwardprojection_sparse.fut:26-40, futhark/backprojection_sparse.fut:26-37):

@@ -44,6 +44,9 @@ SIGNATURES = {
     "sk_sirt": (_i, [_vp, _vp, _i, _vp, _vp, _i, _vp]),
     "sk_sirt_weights": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sk_sirt_step": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_sirt_batched": (_i, [_vp, _vp, _i, _i, _vp]),
+    "sk_autotune": (_i, [_vp, _vp, _ip, _ip, _vp]),
+    "sk_geom_set_variants": (_i, [_vp, _i, _i]),
     "sk_sparse_build": (_i, [_vp, _vp, _i, ctypes.POINTER(_vp), _vp]),
     "sk_sparse_count": (_i, [_vp, _vp, _i, _i64p, _vp]),
     "sk_mat_info": (_i, [_vp, _i64p, _i64p, _i64p]),
@@ -52,6 +55,11 @@ SIGNATURES = {
     "sk_spmv": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_mat_destroy": (None, [_vp]),
 }
+
+
+class SkProblem(ctypes.Structure):
+    """sk_problem (include/synkro.h)."""
+    _fields_ = [("geom", _vp), ("batch", _i), ("d_img", _vp), ("d_sino", _vp)]
 
 
 class NativeMissing(RuntimeError):

@@ -1,0 +1,60 @@
+"""Kernel-variant autotuning with a persistent cache.
+
+``tuned(geom)`` applies the cached choice for the geometry's shape (image size, angle
+count, bins, bin width, device architecture) or runs ``sk_autotune`` on the GPU and stores
+the result.  The cache is a JSON file at $SYNKRO_TUNE_CACHE, default
+~/.cache/synkrotomo_amd/tune.json.  Variants only change tile shapes and rays per lane --
+every variant computes identical results -- so a stale cache can only cost time.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import torch
+
+from .projector import Geometry
+
+
+def cache_path() -> str:
+    return os.environ.get("SYNKRO_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "synkrotomo_amd", "tune.json"))
+
+
+def _key(geom: Geometry) -> str:
+    arch = torch.cuda.get_device_properties(geom.ctx.device).gcnArchName.split(":")[0]
+    return f"{arch}/N{geom.N}/A{geom.A}/R{geom.R}/d{geom.drho:g}"
+
+
+def _load() -> dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict) -> None:
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # read-only home: tuning still applies to this process
+
+
+def tuned(geom: Geometry, retune: bool = False) -> tuple[int, int]:
+    """Apply (and if needed measure) the fastest kernel variants for this geometry."""
+    table = _load()
+    key = _key(geom)
+    if not retune and key in table:
+        fv, bv = table[key]
+        geom.set_variants(fv, bv)
+        return fv, bv
+    fv, bv = geom.autotune()
+    table[key] = [fv, bv]
+    _store(table)
+    return fv, bv

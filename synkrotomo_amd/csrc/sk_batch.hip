@@ -1,0 +1,320 @@
+// sk_batch.hip -- many SIRT problems of uneven size in persistent launches, and the
+// kernel-variant autotuner.
+//
+// Batched SIRT (no reference counterpart; generalizes SIRT3D.fut:18-28, which maps SIRT over
+// the slices of one geometry): a problem set holds groups of independent problems, each
+// group with its own geometry (image size, angles, bins).  One iteration is two persistent
+// launches sized to fill the chip once: k_fp_persist pulls (problem, line, bin-block) items
+// from a global work counter and k_bp_persist pulls (problem, tile) items from another.
+// Items are queued largest-problem first so the tail of the queue is made of the shortest
+// items.  Each kernel resets the other's counter, so the whole iteration sequence (a reset
+// kernel, then fp/bp pairs) is captured once into a hipGraph and launched once.
+//
+// Autotuner: times the fp variants (1 or 2 rays per lane) and the bp variants (16x16,
+// 16x32 or 16x64 pixel tiles) on a geometry with HIP events and keeps the fastest.  All
+// variants evaluate the same expressions in the same order per ray and per pixel, so the
+// choice never changes results.
+#include <algorithm>
+#include <vector>
+
+#include "sk_kern.h"
+
+namespace sk {
+
+int launched(Ctx* c, const char* what);
+int ensure_weights(Ctx* c, Geom* g, hipStream_t s);
+FpParams fp_params(const Geom* g, const FpArgs& a);
+BpParams bp_params(const Geom* g, const BpArgs& a, int ppt);
+
+struct BatchProb {
+  FpParams fp;
+  BpParams bp;
+  int batch;
+  int fp_bx;          // bin blocks per line
+  int bp_tx, bp_ty;   // tiles across / down
+  long fp_begin, bp_begin;
+};
+
+__global__ void k_reset_counters(unsigned* ctr) {
+  if (threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+}
+
+// Uniform (per block) lookup of the problem owning global item `item`.
+template <bool FP>
+__device__ __forceinline__ int find_prob(const BatchProb* probs, int nprob, long item) {
+  int p = 0;
+  while (p + 1 < nprob && item >= (FP ? probs[p + 1].fp_begin : probs[p + 1].bp_begin)) ++p;
+  return p;
+}
+
+template <bool EXACT, int RPT>
+__global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchProb* __restrict__ probs,
+                                                         int nprob, long total,
+                                                         unsigned* __restrict__ ctr) {
+  __shared__ unsigned item_sh;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0u;  // the bp queue of this iteration
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[0], 1u);
+    __syncthreads();
+    const long item = item_sh;
+    __syncthreads();
+    if (item >= total) break;
+    const int p = find_prob<true>(probs, nprob, item);
+    const BatchProb& B = probs[p];
+    const long local = item - B.fp_begin;
+    const long per_img = (long)B.fp.A * B.fp_bx;
+    const int bz = (int)(local / per_img);
+    const long rem = local - (long)bz * per_img;
+    const int j = (int)(rem / B.fp_bx);
+    const int bx = (int)(rem - (long)j * B.fp_bx);
+    fp_item<EXACT, FP_SIRT, W_REF, true, RPT>(B.fp, j, bx, bz);
+  }
+}
+
+template <bool EXACT, int NC, bool DR1, int PPT>
+__global__ __launch_bounds__(256) void k_bp_persist(const BatchProb* __restrict__ probs,
+                                                    int nprob, long total,
+                                                    unsigned* __restrict__ ctr) {
+  extern __shared__ float2 win[];
+  __shared__ int w0s[BP_LB];
+  __shared__ unsigned item_sh;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0u;  // the fp queue of the next iteration
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[1], 1u);
+    __syncthreads();
+    const long item = item_sh;
+    __syncthreads();
+    if (item >= total) break;
+    const int p = find_prob<false>(probs, nprob, item);
+    const BatchProb& B = probs[p];
+    const long local = item - B.bp_begin;
+    const long per_img = (long)B.bp_tx * B.bp_ty;
+    const int bz = (int)(local / per_img);
+    const long rem = local - (long)bz * per_img;
+    const int ty = (int)(rem / B.bp_tx);
+    const int tx = (int)(rem - (long)ty * B.bp_tx);
+    bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, tx, ty, bz, win, w0s);
+  }
+}
+
+struct BatchPlan {
+  std::vector<BatchProb> host;
+  BatchProb* d_probs = nullptr;
+  unsigned* d_ctr = nullptr;
+  long fp_total = 0, bp_total = 0;
+  int Wmax = 0;
+  bool dr1 = true;
+};
+
+constexpr int BATCH_RPT = 1;
+constexpr int BATCH_PPT = 2;
+
+template <bool EXACT, int NC, bool DR1>
+static int launch_iteration(Ctx* c, const BatchPlan& plan, int fp_grid, int bp_grid, hipStream_t s) {
+  const int np = (int)plan.host.size();
+  hipLaunchKernelGGL((k_fp_persist<EXACT, BATCH_RPT>), dim3(fp_grid), dim3(FP_BLOCK), 0, s,
+                     plan.d_probs, np, plan.fp_total, plan.d_ctr);
+  const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
+  hipLaunchKernelGGL((k_bp_persist<EXACT, NC, DR1, BATCH_PPT>), dim3(bp_grid), dim3(256), lds, s,
+                     plan.d_probs, np, plan.bp_total, plan.d_ctr);
+  return launched(c, "persistent SIRT kernels");
+}
+
+static int persistent_grid(Ctx* c, const void* kernel, int block, size_t lds, int* out) {
+  int dev = c->device, cus = 0, per_cu = 0;
+  SK_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  SK_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds));
+  *out = std::max(1, cus * std::max(1, per_cu));
+  return SK_OK;
+}
+
+}  // namespace sk
+
+using namespace sk;
+
+extern "C" {
+
+int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream) {
+  if (!ctx) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  if (!problems || count < 1) return fail(c, SK_EINVAL, "need at least one problem group");
+  if (iters < 0) return fail(c, SK_EINVAL, "iterations must be >= 0 (got %d)", iters);
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < count; i++) {
+    const sk_problem& P = problems[i];
+    if (!P.geom || !P.d_img || !P.d_sino) return fail(c, SK_EINVAL, "problem %d: NULL argument", i);
+    if (P.batch < 1) return fail(c, SK_EINVAL, "problem %d: batch must be >= 1", i);
+    if (P.geom->g.device != c->device) return fail(c, SK_EINVAL, "problem %d: other device", i);
+  }
+  if (iters == 0) return SK_OK;
+  for (int i = 0; i < count; i++) {
+    int rc = ensure_weights(c, &problems[i].geom->g, s);
+    if (rc) return rc;
+  }
+
+  // Largest problems first (their items are the longest).
+  std::vector<int> order(count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const Geom &ga = problems[a].geom->g, &gb = problems[b].geom->g;
+    return (long)ga.N * ga.A > (long)gb.N * gb.A;
+  });
+
+  // Scratch: descriptors | counters | per group: x^T, q
+  size_t ws_floats = 0;
+  for (int i = 0; i < count; i++) {
+    const Geom& G = problems[i].geom->g;
+    ws_floats += (size_t)problems[i].batch * ((size_t)G.N * G.N + (size_t)G.A * G.R);
+  }
+  const size_t desc_bytes = ((sizeof(BatchProb) * count + 255) / 256) * 256;
+  void* ws = nullptr;
+  int rc = scratch(c, desc_bytes + 256 + sizeof(float) * ws_floats, s, &ws);
+  if (rc) return rc;
+  BatchPlan plan;
+  plan.d_probs = (BatchProb*)ws;
+  plan.d_ctr = (unsigned*)((char*)ws + desc_bytes);
+  float* fws = (float*)((char*)ws + desc_bytes + 256);
+  for (int oi = 0; oi < count; oi++) {
+    const sk_problem& P = problems[order[oi]];
+    const Geom& G = P.geom->g;
+    float* imgT = fws;
+    fws += (size_t)P.batch * G.N * G.N;
+    float* q = fws;
+    fws += (size_t)P.batch * G.A * G.R;
+    BatchProb B{};
+    B.fp = fp_params(&G, FpArgs{P.d_img, imgT, q, P.d_sino, G.d_rowinv, P.batch});
+    B.bp = bp_params(&G, BpArgs{q, P.d_img, imgT, G.d_colinv, P.batch}, BATCH_PPT);
+    B.batch = P.batch;
+    B.fp_bx = (G.R + FP_BLOCK * BATCH_RPT - 1) / (FP_BLOCK * BATCH_RPT);
+    B.bp_tx = (G.N + BP_TW - 1) / BP_TW;
+    B.bp_ty = (G.N + 16 * BATCH_PPT - 1) / (16 * BATCH_PPT);
+    B.fp_begin = plan.fp_total;
+    B.bp_begin = plan.bp_total;
+    plan.fp_total += (long)P.batch * G.A * B.fp_bx;
+    plan.bp_total += (long)P.batch * B.bp_tx * B.bp_ty;
+    plan.Wmax = std::max(plan.Wmax, B.bp.W);
+    plan.dr1 = plan.dr1 && G.drho == 1.0f && G.ncand == 2;
+    plan.host.push_back(B);
+    rc = launch_transpose(c, P.d_img, imgT, G.N, P.batch, s);
+    if (rc) return rc;
+  }
+  if (plan.fp_total >= (1L << 32) || plan.bp_total >= (1L << 32))
+    return fail(c, SK_EINVAL, "problem set too large for one work queue");
+  if (sizeof(float2) * BP_LB * (size_t)plan.Wmax > 60 * 1024)
+    return fail(c, SK_EINVAL, "backprojection window exceeds LDS");
+  SK_HIP(c, hipMemcpyAsync(plan.d_probs, plan.host.data(), sizeof(BatchProb) * count,
+                           hipMemcpyHostToDevice, s));
+  SK_HIP(c, hipStreamSynchronize(s));  // host descriptors are read before they go out of scope
+
+  const bool ex = c->exact != 0;
+  const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
+  int fp_grid = 0, bp_grid = 0;
+  const void* kfp = ex ? (const void*)k_fp_persist<true, BATCH_RPT> : (const void*)k_fp_persist<false, BATCH_RPT>;
+  const void* kbp =
+      plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, BATCH_PPT> : (const void*)k_bp_persist<false, 2, true, BATCH_PPT>)
+               : (ex ? (const void*)k_bp_persist<true, 0, false, BATCH_PPT> : (const void*)k_bp_persist<false, 0, false, BATCH_PPT>);
+  rc = persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
+  if (!rc) rc = persistent_grid(c, kbp, 256, lds, &bp_grid);
+  if (rc) return rc;
+
+  // Capture reset + iters x (fp, bp) once, launch once.
+  hipStream_t cs = c->capture_stream;
+  SK_HIP(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, cs, plan.d_ctr);
+  rc = launched(c, "counter reset");
+  for (int it = 0; it < iters && rc == SK_OK; it++) {
+    if (plan.dr1)
+      rc = ex ? launch_iteration<true, 2, true>(c, plan, fp_grid, bp_grid, cs)
+              : launch_iteration<false, 2, true>(c, plan, fp_grid, bp_grid, cs);
+    else
+      rc = ex ? launch_iteration<true, 0, false>(c, plan, fp_grid, bp_grid, cs)
+              : launch_iteration<false, 0, false>(c, plan, fp_grid, bp_grid, cs);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(cs, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+  e = hipGraphLaunch(exec, s);
+  // The executable graph is released after the work it enqueued has finished.
+  hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipGraphExecDestroy(exec);
+  if (e != hipSuccess) return hip_fail(c, e, "hipGraphLaunch");
+  if (e2 != hipSuccess) return hip_fail(c, e2, "batched SIRT");
+  return SK_OK;
+}
+
+int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
+  if (!g) return SK_EINVAL;
+  if (fp_variant < -1 || fp_variant > 1 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
+  g->g.fp_variant = fp_variant;
+  g->g.bp_variant = bp_variant;
+  return SK_OK;
+}
+
+int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream) {
+  if (!ctx) return SK_EINVAL;
+  Ctx* c = &ctx->c;
+  if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
+  SK_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  Geom& G = g->g;
+  const size_t nimg = (size_t)G.N * G.N, nsino = (size_t)G.A * G.R;
+  // own buffers (the projector calls below use the context scratch for x^T)
+  float *img = nullptr, *sino = nullptr;
+  SK_HIP(c, hipMalloc(&img, sizeof(float) * (nimg + nsino)));
+  sino = img + nimg;
+  hipError_t e = hipMemsetAsync(img, 0, sizeof(float) * (nimg + nsino), s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int rc = (e == hipSuccess) ? SK_OK : hip_fail(c, e, "autotune setup");
+  const int saved_fp = G.fp_variant, saved_bp = G.bp_variant;
+  int best_fp = 0, best_bp = 0;
+  auto time_it = [&](bool fp, int v, float* ms) -> int {
+    if (fp) G.fp_variant = v; else G.bp_variant = v;
+    const int reps = 3;
+    for (int k = 0; k <= reps; k++) {
+      if (k == 1) SK_HIP(c, hipEventRecord(e0, s));
+      int r = fp ? sk_fp(ctx, g, 1, img, sino, stream) : sk_bp(ctx, g, 1, sino, img, stream);
+      if (r) return r;
+    }
+    SK_HIP(c, hipEventRecord(e1, s));
+    SK_HIP(c, hipEventSynchronize(e1));
+    SK_HIP(c, hipEventElapsedTime(ms, e0, e1));
+    return SK_OK;
+  };
+  float best = 1e30f, ms = 0.0f;
+  for (int v = 0; v <= 1 && !rc; v++) {
+    rc = time_it(true, v, &ms);
+    if (!rc && ms < best) { best = ms; best_fp = v; }
+  }
+  best = 1e30f;
+  for (int v = 0; v <= 2 && !rc; v++) {
+    rc = time_it(false, v, &ms);
+    if (!rc && ms < best) { best = ms; best_bp = v; }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(img);
+  if (rc) {
+    G.fp_variant = saved_fp;
+    G.bp_variant = saved_bp;
+    return rc;
+  }
+  G.fp_variant = best_fp;
+  G.bp_variant = best_bp;
+  if (fp_variant) *fp_variant = best_fp;
+  if (bp_variant) *bp_variant = best_bp;
+  return SK_OK;
+}
+
+}  // extern "C"

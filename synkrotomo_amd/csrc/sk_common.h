@@ -38,6 +38,8 @@ struct Geom {
   float* d_rowinv = nullptr;  // [A][R]
   float* d_colinv = nullptr;  // [N][N]
   int weights_exact = -1;     // SK_OPT_EXACT value the weights were computed with
+  int fp_variant = -1;        // tuned kernel variants (-1: the context's default)
+  int bp_variant = -1;
   int device = 0;
   // Cached SIRT iteration graph and the arguments it was captured with.
   hipGraphExec_t sirt_exec = nullptr;
@@ -95,7 +97,10 @@ struct BpArgs {
   int batch;
 };
 
+int launched(Ctx* c, const char* what);
 int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipStream_t s);
+int fp_rays_per_lane(int variant);
+int bp_pixels_per_lane(int variant);
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s);
 int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s);
 int launch_bp(Ctx* c, const Geom* g, BpMode mode, BpWeights w, const BpArgs& a, hipStream_t s);

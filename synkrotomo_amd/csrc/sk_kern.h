@@ -1,0 +1,409 @@
+// sk_kern.h -- device code of the projector: one work item of A.x (a block of rays of one
+// line) and one work item of B.r (a pixel tile), shared by the per-call kernels
+// (sk_proj.hip) and the persistent multi-problem kernels (sk_batch.hip).
+//
+// Arithmetic follows the reference expression by expression; this code is compiled with
+// -ffp-contract=off so every multiply and add stays a separate IEEE operation wherever a
+// pixel index or candidate index depends on it.
+#pragma once
+
+#include "sk_common.h"
+
+namespace sk {
+
+constexpr int FP_BLOCK = 256;              // threads per fp work item
+constexpr int FP_WAVES = FP_BLOCK / 64;
+constexpr int BP_TW = 16;                  // pixel tile width; height = 16 * PPT
+constexpr int BP_LB = 32;                  // lines per staged LDS block
+constexpr unsigned BUF_DWORD3 = 0x00020000u;  // gfx9 raw buffer descriptor word 3
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// A buffer descriptor covering one image row of n floats: loads at columns outside [0, n)
+// return 0 (hardware range check), which is exactly the reference's contribution of an
+// out-of-bounds pixel.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const float* row, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, n * 4, BUF_DWORD3);
+}
+__device__ __forceinline__ float row_load(__amdgpu_buffer_rsrc_t r, int col) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, col << 2, 0, 0));
+}
+
+// Conservative frame-row range [i0, i1) outside which both candidate pixels of a ray with
+// this slope and intercept are out of bounds (|x| > h + 2 with a 2-row margin).  Both ends
+// move monotonically, in the same direction, with the intercept, so the hull of the ranges
+// of a wave's first and last rays covers every ray between them.  Near-zero slopes
+// (theta ~ 0, pi/2) take the full range: there a ray is either in for every row or in for
+// none, and an "empty" range would break that monotonicity.
+__device__ __forceinline__ void row_clip(float slope, float icpt, int h, int& i0, int& i1) {
+  const float lo = -(float)h - 2.0f, hi = (float)h + 2.0f;
+  if (fabsf(slope) < 1e-6f) {
+    i0 = -h;
+    i1 = h;
+    return;
+  }
+  const float ta = (lo - icpt) / slope, tb = (hi - icpt) / slope;
+  float tmin = fminf(ta, tb), tmax = fmaxf(ta, tb);
+  tmin = fmaxf(tmin, -(float)h - 4.0f);
+  tmax = fminf(tmax, (float)h + 4.0f);
+  i0 = max(-h, (int)floorf(tmin) - 2);
+  i1 = min(h, (int)ceilf(tmax) + 3);
+}
+
+// First candidate ray of a pixel (backprojection.fut:51-54).
+template <bool DR1>
+__device__ __forceinline__ float cand_s(const Line& L, float xmin, float ymin, float rho0,
+                                        float drho) {
+  const float px = (xmin + 0.5f) - L.fc;
+  const float py = (ymin + 0.5f) - L.fs;
+  const float rho = L.cost * px + L.sint * py;
+  float t = rho - rho0;
+  if constexpr (!DR1) t = t / drho;
+  return ceilf(t);
+}
+
+// Length B[pixel, ray] of the reference backprojector for the pixel in frame column X of
+// frame row y (= the ray's row): intersect_fact of the ray's [xbot, xtop] with [X, X+1),
+// if the ray is one of the pixel's candidates s..s+ncand-1 (backprojection.fut:50-62).
+template <bool EXACT, bool DR1>
+__device__ __forceinline__ float bp_entry(const Line& L, int X, float y, float xbot, float xtop,
+                                          int r, int ncand, float rho0, float drho) {
+  const float xmin = (float)X;
+  const float s = cand_s<DR1>(L, xmin, y, rho0, drho);
+  const int si = (int)s;
+  const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
+  const float lo = fmaxf(maxx - xmin, 0.0f);
+  const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
+  const float diff = maxx - minx;
+  const float m = fminf(lo, hi);
+  float p;
+  if constexpr (EXACT)
+    p = (m == 0.0f) ? 0.0f : m / diff;
+  else
+    p = (m == 0.0f) ? 0.0f : m * fast_rcp(diff);
+  const float l = fminf(p, 1.0f) * L.lbase;
+  return (r >= si && r < si + ncand) ? l : 0.0f;
+}
+
+// Length of one candidate ray in one pixel, in the reference's own formulation.
+//   W_REF: intersect_fact(max, min, xmin, xmin+1) * lbase (backprojection.fut:22-33,59-62)
+//   W_ADJ: the forward projector's length for this (ray, row) if its Xb or Xt is this
+//          column (forwardprojection.fut:37-54), i.e. the entry of A.
+template <bool EXACT, int WEIGHTS>
+__device__ __forceinline__ float cand_len(float xbot, float slope, float lbase, float xmin,
+                                          float xmin1, bool degen) {
+  const float xtop = xbot + slope;
+  if constexpr (WEIGHTS == W_REF) {
+    const float maxx = fmaxf(xbot, xtop);
+    const float minx = fminf(xbot, xtop);
+    const float diff = maxx - minx;
+    float p;
+    if constexpr (EXACT) {
+      const float lo = fmaxf(maxx - xmin, 0.0f);
+      const float hi = fmaxf(xmin1 - minx, 0.0f);
+      const float m = fminf(lo, hi);
+      p = (m == 0.0f) ? 0.0f : m / diff;
+    } else {
+      // min(max(a,0), max(b,0)) == max(min(a,b), 0)
+      const float m = fmaxf(fminf(maxx - xmin, xmin1 - minx), 0.0f);
+      p = m * fast_rcp(diff);  // diff > 0 on non-degenerate lines, where m == 0 gives 0
+      if (degen) p = (m == 0.0f) ? 0.0f : p;
+    }
+    return fminf(p, 1.0f) * lbase;
+  } else {
+    const float fXb = floorf(xbot);
+    const float fXt = floorf(xtop);
+    const bool single = fXb == fXt;
+    const float Xmax = fmaxf(fXb, fXt);
+    const float xdiff = xtop - xbot;
+    float lb, lt;
+    if constexpr (EXACT) {
+      lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
+      lt = ((xtop - Xmax) / xdiff) * lbase;
+    } else {
+      const float rd = fast_rcp(xdiff);
+      lb = single ? lbase : ((Xmax - xbot) * rd) * lbase;
+      lt = ((xtop - Xmax) * rd) * lbase;
+    }
+    return (fXb == xmin) ? lb : ((!single && fXt == xmin) ? lt : 0.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------- A.x item
+struct FpParams {
+  const Line* lines;
+  int A, R, N, ncand;
+  float rho0, drho;
+  const float* img;     // [batch][N][N]
+  const float* imgT;    // transposed copy (flat lines)
+  float* out;           // [batch][A][R]
+  const float* b;       // FP_SIRT: measured sinogram
+  const float* rowinv;  // FP_SIRT: [A][R]
+};
+
+// One work item: line j, bins [bx*FP_BLOCK*RPT, (bx+1)*FP_BLOCK*RPT), image bz.  Each wave
+// owns 64*RPT consecutive bins (lane l: bins first + l + 64k), all lanes stepping the same
+// frame row.  WEIGHTS == W_ADJ walks the same rays with the backprojector's lengths (B^T).
+template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
+__device__ __forceinline__ void fp_item(const FpParams& P, int j, int bx, int bz) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int first = (bx * FP_WAVES + wave) * 64 * RPT;
+  if (first >= P.R) return;  // whole wave past the last bin (no barriers follow)
+  const int last = min(first + 64 * RPT - 1, P.R - 1);
+  const Line L = P.lines[j];
+  const int N = P.N, h = N >> 1;
+  const float cost = L.cost, slope = L.slope, lbase = L.lbase;
+
+  int r[RPT];
+  float icpt[RPT], sum[RPT];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    r[k] = first + 64 * k + lane;
+    icpt[k] = (P.rho0 + (float)r[k] * P.drho) / cost;  // forwardprojection.fut:33-34
+    sum[k] = 0.0f;
+  }
+  int i0, i1;
+  {
+    int a0, a1, b0, b1;
+    row_clip(slope, (P.rho0 + (float)first * P.drho) / cost, h, a0, a1);
+    row_clip(slope, (P.rho0 + (float)last * P.drho) / cost, h, b0, b1);
+    i0 = __builtin_amdgcn_readfirstlane(min(a0, b0));
+    i1 = __builtin_amdgcn_readfirstlane(max(a1, b1));
+  }
+  const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+
+  for (int i = i0; i < i1; ++i) {
+    const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, N);
+    const float yslope = (float)i * slope;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      // forwardprojection.fut:37-56
+      const float xbot = yslope + icpt[k];
+      const float xtop = xbot + slope;
+      const float fXb = floorf(xbot);
+      const float fXt = floorf(xtop);
+      const int Xb = (int)fXb;
+      const int Xt = (int)fXt;
+      const bool single = Xb == Xt;
+      float vb = 1.0f, vt = 1.0f;
+      if constexpr (MODE != FP_ONES_INV) {
+        vb = row_load(row, Xb + h);
+        vt = row_load(row, Xt + h);
+      }
+      if constexpr (WEIGHTS == W_ADJ || EXACT || MODE == FP_ONES_INV) {
+        float lb, lt;
+        if constexpr (WEIGHTS == W_ADJ) {
+          lb = bp_entry<EXACT, DR1>(L, Xb, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
+          lt = bp_entry<EXACT, DR1>(L, Xt, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
+        } else {
+          const float Xmax = fmaxf(fXb, fXt);
+          const float xdiff = xtop - xbot;
+          if constexpr (EXACT) {
+            lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
+            lt = ((xtop - Xmax) / xdiff) * lbase;
+          } else {
+            const float rd = fast_rcp(xdiff);
+            lb = single ? lbase : ((Xmax - xbot) * rd) * lbase;
+            lt = ((xtop - Xmax) * rd) * lbase;
+          }
+        }
+        const bool inb = (unsigned)(Xb + h) < (unsigned)N;
+        const bool intop = !single && (unsigned)(Xt + h) < (unsigned)N;
+        const float bot = inb ? lb * vb : 0.0f;
+        const float top = intop ? lt * vt : 0.0f;
+        sum[k] = sum[k] + (bot + top);
+      } else {
+        // Fast mode: the fractions of the row's length (1 in a single pixel), scaled by
+        // lbase once per ray.  Out-of-bounds columns load 0.
+        const float Xmax = fmaxf(fXb, fXt);
+        const float rd = fast_rcp(xtop - xbot);
+        const float fb = single ? 1.0f : (Xmax - xbot) * rd;
+        const float ft = single ? 0.0f : (xtop - Xmax) * rd;
+        sum[k] = fmaf(fb, vb, sum[k]);
+        sum[k] = fmaf(ft, vt, sum[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    if (r[k] >= P.R) continue;
+    float s = sum[k];
+    if constexpr (!(WEIGHTS == W_ADJ || EXACT || MODE == FP_ONES_INV)) s = s * lbase;
+    const size_t o = (size_t)bz * P.A * P.R + (size_t)L.orig * P.R + r[k];
+    if constexpr (MODE == FP_PLAIN) {
+      P.out[o] = s;
+    } else if constexpr (MODE == FP_ONES_INV) {
+      P.out[o] = (s == 0.0f) ? 0.0f : 1.0f / s;  // sirtlib.fut:9-10
+    } else {
+      // SIRT.fut:33-36: R * (b - A x)
+      P.out[o] = P.rowinv[(size_t)L.orig * P.R + r[k]] * (P.b[o] - s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------- B.r item
+struct BpParams {
+  const Line* lines;
+  int A, R, N, ncand, W;
+  float rho0, drho;
+  const float* sino;    // [batch][A][R]
+  float* out;           // [batch][N][N]  (BP_SIRT: the image, updated in place)
+  float* outT;          // BP_SIRT: transposed copy of the updated image
+  const float* colinv;  // BP_SIRT: [N][N]
+};
+
+// LDS window width (bins) covering every candidate of a BP_TW x (16*PPT) tile.
+__host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
+  const double w = BP_TW - 1, hgt = 16.0 * ppt - 1;
+  const double span = sqrt(w * w + hgt * hgt) * 1.00001 / (double)drho;
+  const int W = (int)ceil(span) + ncand + 3;
+  return (W + 1) & ~1;
+}
+
+// One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
+// (ty, tx) owns pixels (Y0 + ty + 16k, X0 + tx).  Lines are walked in order in blocks of
+// BP_LB: the block finds each line's rho-window origin from the tile's corners (the
+// candidate index is monotone in each frame coordinate), stages the window -- (s*drho +
+// rho0)/cost and the sinogram value -- in LDS, then every lane evaluates its candidates
+// from LDS.  Steep and flat lines accumulate separately and are added at the end
+// (backprojection.fut:75-78).  win: BP_LB*W float2, w0s: BP_LB ints, both in LDS.
+template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
+__device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_y, int bz,
+                                        float2* win, int* w0s) {
+  // Fast reference weights: slope sign hoisted per line, lbase folded into the staged value.
+  constexpr bool FASTREF = !EXACT && WEIGHTS == W_REF;
+  const int ncand = NC > 0 ? NC : P.ncand;
+  const int W = P.W;
+  const int tx = threadIdx.x & (BP_TW - 1), ty = threadIdx.x / BP_TW;
+  constexpr int TH = 16 * PPT;
+  const int X0 = tile_x * BP_TW, Y0 = tile_y * TH;
+  const int N = P.N, h = N >> 1;
+  const float* sb = P.sino + (size_t)bz * P.A * P.R;
+  const int X = X0 + tx;
+  float xs[PPT], ys[PPT], acc_s[PPT], acc_f[PPT];
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    xs[k] = (float)(X - h);
+    ys[k] = (float)(Y0 + ty + 16 * k - h);
+    acc_s[k] = 0.0f;
+    acc_f[k] = 0.0f;
+  }
+
+  for (int l0 = 0; l0 < P.A; l0 += BP_LB) {
+    const int nl = min(BP_LB, P.A - l0);
+    if ((int)threadIdx.x < nl) {
+      const Line L = P.lines[l0 + threadIdx.x];
+      const bool fl = L.flags & LINE_FLAT;
+      const float cx0 = (float)((fl ? Y0 : X0) - h), cy0 = (float)((fl ? X0 : Y0) - h);
+      const float cx1 = cx0 + (float)((fl ? TH : BP_TW) - 1);
+      const float cy1 = cy0 + (float)((fl ? BP_TW : TH) - 1);
+      float smin = cand_s<DR1>(L, cx0, cy0, P.rho0, P.drho);
+      smin = fminf(smin, cand_s<DR1>(L, cx1, cy0, P.rho0, P.drho));
+      smin = fminf(smin, cand_s<DR1>(L, cx0, cy1, P.rho0, P.drho));
+      smin = fminf(smin, cand_s<DR1>(L, cx1, cy1, P.rho0, P.drho));
+      w0s[threadIdx.x] = (int)smin;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nl * W; e += 256) {
+      const int jj = e / W, k = e - jj * W;
+      const Line L = P.lines[l0 + jj];
+      const int sp = w0s[jj] + k;
+      // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost
+      const float inter = ((float)sp * P.drho + P.rho0) / L.cost;
+      float p = 0.0f;
+      if (sp >= 0 && sp < P.R) p = (MODE == BP_ONES_INV) ? 1.0f : sb[(size_t)L.orig * P.R + sp];
+      if constexpr (FASTREF) p = p * L.lbase;
+      win[jj * W + k] = make_float2(inter, p);
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nl; ++jj) {
+      const Line L = P.lines[l0 + jj];
+      const bool fl = L.flags & LINE_FLAT;
+      const bool degen = L.flags & LINE_DEGEN;
+      const bool pos = L.flags & LINE_SLOPE_POS;
+      const int w0 = w0s[jj];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const float xmin = fl ? ys[q] : xs[q];
+        const float ymin = fl ? xs[q] : ys[q];
+        const float s = cand_s<DR1>(L, xmin, ymin, P.rho0, P.drho);
+        const float xbase = ymin * L.slope;  // backprojection.fut:55
+        // s - w0 lies in [0, W - ncand] (monotone s, window sized on the host); the clamp
+        // only guards geometries so far off-centre that rho rounds by whole bins.
+        const int off = min(max((int)s - w0, 0), W - ncand);
+        const float2* wrow = win + jj * W + off;
+        const float xmin1 = xmin + 1.0f;
+        float inner = 0.0f;
+        if (FASTREF && !degen) {
+          if (pos) {
+#pragma unroll
+            for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
+              if (NC == 0 && k >= ncand) break;
+              const float2 e = wrow[k];
+              const float xb = xbase + e.x;
+              const float xt = xb + L.slope;
+              const float m = fmaxf(fminf(xt - xmin, xmin1 - xb), 0.0f);
+              const float f = fminf(m * fast_rcp(xt - xb), 1.0f);
+              inner = fmaf(f, e.y, inner);
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
+              if (NC == 0 && k >= ncand) break;
+              const float2 e = wrow[k];
+              const float xb = xbase + e.x;
+              const float xt = xb + L.slope;
+              const float m = fmaxf(fminf(xb - xmin, xmin1 - xt), 0.0f);
+              const float f = fminf(m * fast_rcp(xb - xt), 1.0f);
+              inner = fmaf(f, e.y, inner);
+            }
+          }
+        } else if (FASTREF) {
+          // degenerate line: general form with the m == 0 case, lbase already in e.y
+#pragma unroll
+          for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
+            if (NC == 0 && k >= ncand) break;
+            const float2 e = wrow[k];
+            const float l = cand_len<false, W_REF>(xbase + e.x, L.slope, 1.0f, xmin, xmin1, true);
+            inner = fmaf(l, e.y, inner);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
+            if (NC == 0 && k >= ncand) break;
+            const float2 e = wrow[k];
+            const float l =
+                cand_len<EXACT, WEIGHTS>(xbase + e.x, L.slope, L.lbase, xmin, xmin1, degen);
+            inner = inner + l * e.y;
+          }
+        }
+        if (fl)
+          acc_f[q] = acc_f[q] + inner;
+        else
+          acc_s[q] = acc_s[q] + inner;
+      }
+    }
+    __syncthreads();
+  }
+  if (X >= N) return;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int Y = Y0 + ty + 16 * q;
+    if (Y >= N) continue;
+    const size_t p = (size_t)bz * N * N + (size_t)Y * N + X;
+    const float v = acc_s[q] + acc_f[q];  // backprojection.fut:78
+    if constexpr (MODE == BP_PLAIN) {
+      P.out[p] = v;
+    } else if constexpr (MODE == BP_ONES_INV) {
+      P.out[p] = (v == 0.0f) ? 0.0f : 1.0f / v;
+    } else {
+      // SIRT.fut:37-39: x + C * B(q)
+      const float xn = P.out[p] + P.colinv[(size_t)Y * N + X] * v;
+      P.out[p] = xn;
+      P.outT[(size_t)bz * N * N + (size_t)X * N + Y] = xn;
+    }
+  }
+}
+
+}  // namespace sk

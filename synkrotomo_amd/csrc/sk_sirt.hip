@@ -12,7 +12,7 @@ namespace sk {
 
 // Row and column sums (SIRT.fut:22-25): fp of an all-ones image and bp of an all-ones
 // sinogram, with the safe inverse fused into the kernels' epilogues.
-static int ensure_weights(Ctx* c, Geom* g, hipStream_t s) {
+int ensure_weights(Ctx* c, Geom* g, hipStream_t s) {
   if (g->d_rowinv && g->d_colinv && g->weights_exact == c->exact) return SK_OK;
   if (!g->d_rowinv) SK_HIP(c, hipMalloc(&g->d_rowinv, sizeof(float) * (size_t)g->A * g->R));
   if (!g->d_colinv) SK_HIP(c, hipMalloc(&g->d_colinv, sizeof(float) * (size_t)g->N * g->N));

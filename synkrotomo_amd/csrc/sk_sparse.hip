@@ -259,12 +259,6 @@ __global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr
   if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
 }
 
-static int launched(Ctx* c, const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(c, e, what);
-  return SK_OK;
-}
-
 // counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
 static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr, int64_t* nnz,
                        hipStream_t s) {

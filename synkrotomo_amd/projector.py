@@ -232,6 +232,17 @@ class Geometry:
                                     int(iterations), _stream(self.ctx.device)), self.ctx.handle, "SIRT")
         return img
 
+    def autotune(self):
+        """Time the kernel variants on this geometry, keep the fastest; returns (fp, bp)."""
+        fv, bv = ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.lib().sk_autotune(self.ctx.handle, self.handle, ctypes.byref(fv), ctypes.byref(bv),
+                                        _stream(self.ctx.device)), self.ctx.handle, "autotune")
+        return fv.value, bv.value
+
+    def set_variants(self, fp_variant: int = -1, bp_variant: int = -1) -> None:
+        nat.check(nat.lib().sk_geom_set_variants(self.handle, int(fp_variant), int(bp_variant)),
+                  self.ctx.handle, "set variants")
+
     def sirt_weights(self):
         """(1/rowsum(A) as [A, R], 1/colsum(B) as [N, N]), 0 where the sum is 0."""
         rw = torch.empty((self.A, self.R), dtype=torch.float32, device=self.device)
@@ -306,6 +317,32 @@ class Matrix:
         nat.check(nat.lib().sk_spmv(g.ctx.handle, self.handle, batch, _ptr(x), _ptr(out),
                                     _stream(g.ctx.device)), g.ctx.handle, "SpMV")
         return out
+
+
+def sirt_batched(groups, iterations: int, ctx: Context | None = None) -> None:
+    """Advance every problem ``iterations`` SIRT iterations in one batched launch sequence.
+
+    groups: iterable of (geometry, images [b, N, N], sinograms [b, A, R]) -- each group any
+    number of independent problems of one geometry; images are updated in place."""
+    groups = list(groups)
+    if not groups:
+        raise ValueError("no problem groups")
+    ctx = ctx or groups[0][0].ctx
+    arr = (nat.SkProblem * len(groups))()
+    keep = []
+    for k, (geom, img, sino) in enumerate(groups):
+        if geom.ctx is not ctx:
+            raise ValueError("all geometries must belong to one context")
+        b = geom._check(img, geom.N * geom.N, "image")
+        if geom._check(sino, geom.A * geom.R, "projections") != b:
+            raise ValueError("image and projections batch sizes differ")
+        arr[k].geom = geom.handle.value
+        arr[k].batch = b
+        arr[k].d_img = img.data_ptr()
+        arr[k].d_sino = sino.data_ptr()
+        keep.append((img, sino))
+    nat.check(nat.lib().sk_sirt_batched(ctx.handle, ctypes.cast(arr, ctypes.c_void_p), len(groups),
+                                        int(iterations), _stream(ctx.device)), ctx.handle, "batched SIRT")
 
 
 def uniform_angles(A: int) -> np.ndarray:

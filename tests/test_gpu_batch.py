@@ -1,0 +1,94 @@
+"""Kernel variants, the autotuner and batched persistent SIRT."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _uniform(A):
+    return (np.arange(A, dtype=np.float64) * np.pi / A).astype(np.float32)
+
+
+@pytest.mark.parametrize("N,A,R,rz,dr", [(64, 90, 91, -45.0, 1.0), (512, 360, 725, -362.0, 1.0),
+                                         (40, 23, 30, -29.0, 2.0), (24, 19, 70, -17.25, 0.5)])
+def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
+    geo = ctx.geometry(_uniform(A), rz, dr, R, N)
+    x = torch.rand(2, N, N, device="cuda")
+    y = torch.rand(2, A, R, device="cuda")
+    geo.set_variants(0, 0)
+    f0, b0 = geo.fp(x), geo.bp(y)
+    for fv in (0, 1):
+        for bv in (0, 1, 2):
+            geo.set_variants(fv, bv)
+            torch.testing.assert_close(geo.fp(x), f0, rtol=0, atol=0)
+            torch.testing.assert_close(geo.bp(y), b0, rtol=0, atol=0)
+    geo.set_variants(-1, -1)
+    if N == 64:
+        ref = oracle.fp(_uniform(A), rz, dr, R, x[0].cpu().numpy())
+        assert rel_l2(f0[0].cpu().numpy(), ref) <= 1e-5
+
+
+def test_autotune_cache(ctx, tmp_path, monkeypatch):
+    from synkrotomo_amd import autotune
+    monkeypatch.setenv("SYNKRO_TUNE_CACHE", str(tmp_path / "tune.json"))
+    geo = ctx.geometry(_uniform(180), -181.0, 1.0, 363, 256)
+    fv, bv = autotune.tuned(geo)
+    assert fv in (0, 1) and bv in (0, 1, 2)
+    assert (tmp_path / "tune.json").exists()
+    assert autotune.tuned(geo) == (fv, bv)  # served from the cache
+    geo.set_variants(-1, -1)
+
+
+def test_sirt_batched_equals_per_problem(ctx):
+    """Uneven groups in one persistent launch sequence == each problem run on its own."""
+    from synkrotomo_amd.projector import sirt_batched
+    specs = [(32, 30, 46, -22.5, 3), (64, 45, 91, -45.0, 2), (48, 20, 68, -33.5, 4)]
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    groups, singles = [], []
+    for N, A, R, rz, b in specs:
+        geo = ctx.geometry(_uniform(A), rz, 1.0, R, N)
+        sino = torch.rand(b, A, R, device="cuda", generator=gen)
+        img = torch.zeros(b, N, N, device="cuda")
+        groups.append((geo, img, sino))
+        ref = torch.zeros(b, N, N, device="cuda")
+        geo.sirt_(ref, sino, 7)
+        singles.append(ref)
+    sirt_batched(groups, 7)
+    for (geo, img, _), ref in zip(groups, singles):
+        torch.testing.assert_close(img, ref, rtol=0, atol=0)
+
+
+def test_sirt_batched_exact_matches_oracle(exact):
+    from synkrotomo_amd.projector import sirt_batched
+    g = load_golden("c1.npz")
+    geo = exact.geometry(g["angles"], -45.0, 1.0, 91, 64)
+    geo2 = exact.geometry(_uniform(12), -11.5, 1.0, 24, 16)
+    b2 = np.random.default_rng(4).random((3, 12, 24)).astype(np.float32)
+    img1 = torch.zeros(1, 64, 64, device="cuda")
+    img2 = torch.zeros(3, 16, 16, device="cuda")
+    sirt_batched([(geo, img1, torch.from_numpy(g["fp"]).cuda().reshape(1, 90, 91)),
+                  (geo2, img2, torch.from_numpy(b2).cuda())], 10)
+    np.testing.assert_array_equal(img1.cpu().numpy().ravel(), g["sirt10"])
+    for i in range(3):
+        ref = oracle.sirt(_uniform(12), -11.5, 1.0, np.zeros(256, np.float32), b2[i], 10)
+        np.testing.assert_array_equal(img2[i].cpu().numpy().ravel(), ref)
+
+
+def test_sirt_batched_mixed_bin_widths(ctx):
+    """Groups with deltarho != 1 take the general-candidate persistent kernel."""
+    from synkrotomo_amd.projector import sirt_batched
+    geo_a = ctx.geometry(_uniform(19), -17.25, 0.5, 70, 24)
+    geo_b = ctx.geometry(_uniform(30), -22.5, 1.0, 46, 32)
+    ya = torch.rand(2, 19, 70, device="cuda")
+    yb = torch.rand(1, 30, 46, device="cuda")
+    xa, xb = torch.zeros(2, 24, 24, device="cuda"), torch.zeros(1, 32, 32, device="cuda")
+    sirt_batched([(geo_a, xa, ya), (geo_b, xb, yb)], 5)
+    for i in range(2):
+        ref = oracle.sirt(_uniform(19), -17.25, 0.5, np.zeros(576, np.float32), ya[i].cpu().numpy(), 5)
+        assert rel_l2(xa[i].cpu().numpy().ravel(), ref) <= 1e-5
+    ref = oracle.sirt(_uniform(30), -22.5, 1.0, np.zeros(1024, np.float32), yb[0].cpu().numpy(), 5)
+    assert rel_l2(xb[0].cpu().numpy().ravel(), ref) <= 1e-5
