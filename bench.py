@@ -134,6 +134,7 @@ def main() -> int:
     ap.add_argument("--exact", action="store_true", help="IEEE division everywhere (bitwise mode)")
     ap.add_argument("--no-sparse", action="store_true", help="skip the explicit-matrix leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-SIRT leg")
+    ap.add_argument("--batched-iters", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use default kernel variants")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device copy first (reference dispatch for HBM counters)")
@@ -224,7 +225,7 @@ def main() -> int:
         "batched": None,
     }
     if not args.no_batched:
-        out["batched"] = batched_leg(ctx)
+        out["batched"] = batched_leg(ctx, args.batched_iters)
     if not args.no_sparse:
         out["sparse"] = sparse_leg(geo, x, sino, max(3, K // 2))
     if not args.no_cpu:

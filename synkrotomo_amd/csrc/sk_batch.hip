@@ -4,17 +4,19 @@
 // Batched SIRT (no reference counterpart; generalizes SIRT3D.fut:18-28, which maps SIRT over
 // the slices of one geometry): a problem set holds groups of independent problems, each
 // group with its own geometry (image size, angles, bins).  One iteration is two persistent
-// launches sized to fill the chip once: k_fp_persist pulls (problem, line, bin-block) items
-// from a global work counter and k_bp_persist pulls (problem, tile) items from another.
-// Items are queued largest-problem first so the tail of the queue is made of the shortest
-// items.  Each kernel resets the other's counter, so the whole iteration sequence (a reset
-// kernel, then fp/bp pairs) is captured once into a hipGraph and launched once.
+// launches sized to fill the chip once: in k_fp_persist every wave walks a static stride of
+// (problem, image, line, 64-bin group) items; in k_bp_persist every block pulls (problem,
+// image, tile) items from a global work counter, queued largest-problem first so the tail
+// of the queue is made of the shortest items.  The fp kernel resets the bp counter and the
+// bp kernel the fp slot, so the whole iteration sequence (a reset kernel, then fp/bp pairs)
+// is captured once into a hipGraph and launched once.
 //
 // Autotuner: times the fp variants (1 or 2 rays per lane) and the bp variants (16x16,
 // 16x32 or 16x64 pixel tiles) on a geometry with HIP events and keeps the fastest.  All
 // variants evaluate the same expressions in the same order per ray and per pixel, so the
 // choice never changes results.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "sk_kern.h"
@@ -30,93 +32,122 @@ struct BatchProb {
   FpParams fp;
   BpParams bp;
   int batch;
-  int fp_bx;          // bin blocks per line
+  int fp_wb;          // wave-sized bin groups per line
   int bp_tx, bp_ty;   // tiles across / down
   long fp_begin, bp_begin;
 };
 
+__device__ __forceinline__ int u32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The problem table travels by value in the kernel arguments (kernarg memory is read with
+// scalar loads), so every per-problem value -- and every row pointer built from it -- stays
+// in scalar registers.
+constexpr int MAX_GROUPS = 16;
+struct BatchArgs {
+  BatchProb p[MAX_GROUPS];
+  int n;
+  long fp_total, bp_total;
+};
+
 __global__ void k_reset_counters(unsigned* ctr) {
-  if (threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+  if (threadIdx.x < 3) ctr[threadIdx.x] = 0u;
 }
 
-// Uniform (per block) lookup of the problem owning global item `item`.
+// Uniform lookup of the problem owning global item `item`.
 template <bool FP>
-__device__ __forceinline__ int find_prob(const BatchProb* probs, int nprob, long item) {
+__device__ __forceinline__ int find_prob(const BatchArgs& a, long item) {
   int p = 0;
-  while (p + 1 < nprob && item >= (FP ? probs[p + 1].fp_begin : probs[p + 1].bp_begin)) ++p;
+  while (p + 1 < a.n && item >= (FP ? a.p[p + 1].fp_begin : a.p[p + 1].bp_begin)) ++p;
   return p;
 }
 
-template <bool EXACT, int RPT>
-__global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchProb* __restrict__ probs,
-                                                         int nprob, long total,
-                                                         unsigned* __restrict__ ctr) {
-  __shared__ unsigned item_sh;
+// Forward items are one wave each (64*RPT bins of one line of one image).  DYN: waves claim
+// CLAIM consecutive items at a time from a global counter (items vary in cost -- rays near
+// the detector edge are clipped short -- and the counter balances them); otherwise items
+// are strided statically over the grid's waves.  No barriers either way.  Consecutive
+// items are adjacent bins of one line, so concurrent gathers share image rows.
+constexpr int CLAIM = 2;
+
+template <bool EXACT, int RPT, bool DYN>
+__global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0u;  // the bp queue of this iteration
+  const long nwaves = (long)gridDim.x * FP_WAVES;
+  const int wave = u32(threadIdx.x >> 6);  // uniform per wave
+  const int lane = threadIdx.x & 63;
+  long item = DYN ? 0 : (long)blockIdx.x * FP_WAVES + wave;
+  int left = 0;
   for (;;) {
-    if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[0], 1u);
-    __syncthreads();
-    const long item = item_sh;
-    __syncthreads();
-    if (item >= total) break;
-    const int p = find_prob<true>(probs, nprob, item);
-    const BatchProb& B = probs[p];
+    if (DYN) {
+      if (left == 0) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(&ctr[2], (unsigned)CLAIM);
+        item = (long)(unsigned)u32((int)base);
+        left = CLAIM;
+      }
+      --left;
+    }
+    if (item >= a.fp_total) break;
+    const BatchProb& B = a.p[find_prob<true>(a, item)];
     const long local = item - B.fp_begin;
-    const long per_img = (long)B.fp.A * B.fp_bx;
+    const long per_img = (long)B.fp.A * B.fp_wb;
     const int bz = (int)(local / per_img);
     const long rem = local - (long)bz * per_img;
-    const int j = (int)(rem / B.fp_bx);
-    const int bx = (int)(rem - (long)j * B.fp_bx);
-    fp_item<EXACT, FP_SIRT, W_REF, true, RPT>(B.fp, j, bx, bz);
+    const int j = (int)(rem / B.fp_wb);
+    const int wb = (int)(rem - (long)j * B.fp_wb);
+    fp_item<EXACT, FP_SIRT, W_REF, true, RPT>(B.fp, u32(j), u32(wb), u32(bz));
+    item += DYN ? 1 : nwaves;
   }
 }
 
 template <bool EXACT, int NC, bool DR1, int PPT>
-__global__ __launch_bounds__(256) void k_bp_persist(const BatchProb* __restrict__ probs,
-                                                    int nprob, long total,
-                                                    unsigned* __restrict__ ctr) {
+__global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
   __shared__ int w0s[BP_LB];
   __shared__ unsigned item_sh;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0u;  // the fp queue of the next iteration
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[2] = 0u;  // the fp queue of the next iteration
   for (;;) {
     if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[1], 1u);
     __syncthreads();
     const long item = item_sh;
     __syncthreads();
-    if (item >= total) break;
-    const int p = find_prob<false>(probs, nprob, item);
-    const BatchProb& B = probs[p];
+    if (item >= a.bp_total) break;
+    const BatchProb& B = a.p[find_prob<false>(a, item)];
     const long local = item - B.bp_begin;
     const long per_img = (long)B.bp_tx * B.bp_ty;
     const int bz = (int)(local / per_img);
     const long rem = local - (long)bz * per_img;
     const int ty = (int)(rem / B.bp_tx);
     const int tx = (int)(rem - (long)ty * B.bp_tx);
-    bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, tx, ty, bz, win, w0s);
+    bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, u32(tx), u32(ty), u32(bz), win, w0s);
   }
 }
 
 struct BatchPlan {
-  std::vector<BatchProb> host;
-  BatchProb* d_probs = nullptr;
+  BatchArgs args{};
+  // bit0: 2 rays per lane, bit1: dynamic claiming.  3 measured fastest on the batched
+  // configuration (static striding leaves waves with the costliest items idle-waiting at
+  // the end); SYNKRO_BATCH_FP_MODE overrides it.
+  int fp_mode = 3;
   unsigned* d_ctr = nullptr;
-  long fp_total = 0, bp_total = 0;
   int Wmax = 0;
   bool dr1 = true;
 };
 
-constexpr int BATCH_RPT = 1;
 constexpr int BATCH_PPT = 2;
 
 template <bool EXACT, int NC, bool DR1>
 static int launch_iteration(Ctx* c, const BatchPlan& plan, int fp_grid, int bp_grid, hipStream_t s) {
-  const int np = (int)plan.host.size();
-  hipLaunchKernelGGL((k_fp_persist<EXACT, BATCH_RPT>), dim3(fp_grid), dim3(FP_BLOCK), 0, s,
-                     plan.d_probs, np, plan.fp_total, plan.d_ctr);
+  if (plan.fp_mode == 0)
+    hipLaunchKernelGGL((k_fp_persist<EXACT, 1, false>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
+  else if (plan.fp_mode == 1)
+    hipLaunchKernelGGL((k_fp_persist<EXACT, 2, false>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
+  else if (plan.fp_mode == 2)
+    hipLaunchKernelGGL((k_fp_persist<EXACT, 1, true>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
+  else
+    hipLaunchKernelGGL((k_fp_persist<EXACT, 2, true>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
   const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
   hipLaunchKernelGGL((k_bp_persist<EXACT, NC, DR1, BATCH_PPT>), dim3(bp_grid), dim3(256), lds, s,
-                     plan.d_probs, np, plan.bp_total, plan.d_ctr);
+                     plan.args, plan.d_ctr);
   return launched(c, "persistent SIRT kernels");
 }
 
@@ -138,6 +169,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   if (!ctx) return SK_EINVAL;
   Ctx* c = &ctx->c;
   if (!problems || count < 1) return fail(c, SK_EINVAL, "need at least one problem group");
+  if (count > MAX_GROUPS) return fail(c, SK_EINVAL, "at most %d problem groups per call", MAX_GROUPS);
   if (iters < 0) return fail(c, SK_EINVAL, "iterations must be >= 0 (got %d)", iters);
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
@@ -161,20 +193,22 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     return (long)ga.N * ga.A > (long)gb.N * gb.A;
   });
 
-  // Scratch: descriptors | counters | per group: x^T, q
+  // Scratch: counters | per group: x^T, q
   size_t ws_floats = 0;
   for (int i = 0; i < count; i++) {
     const Geom& G = problems[i].geom->g;
     ws_floats += (size_t)problems[i].batch * ((size_t)G.N * G.N + (size_t)G.A * G.R);
   }
-  const size_t desc_bytes = ((sizeof(BatchProb) * count + 255) / 256) * 256;
   void* ws = nullptr;
-  int rc = scratch(c, desc_bytes + 256 + sizeof(float) * ws_floats, s, &ws);
+  int rc = scratch(c, 256 + sizeof(float) * ws_floats, s, &ws);
   if (rc) return rc;
   BatchPlan plan;
-  plan.d_probs = (BatchProb*)ws;
-  plan.d_ctr = (unsigned*)((char*)ws + desc_bytes);
-  float* fws = (float*)((char*)ws + desc_bytes + 256);
+  plan.d_ctr = (unsigned*)ws;
+  float* fws = (float*)((char*)ws + 256);
+  BatchArgs& args = plan.args;
+  args.n = count;
+  if (const char* m = getenv("SYNKRO_BATCH_FP_MODE")) plan.fp_mode = atoi(m) & 3;
+  const int rpt = (plan.fp_mode & 1) ? 2 : 1;
   for (int oi = 0; oi < count; oi++) {
     const sk_problem& P = problems[order[oi]];
     const Geom& G = P.geom->g;
@@ -186,37 +220,37 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     B.fp = fp_params(&G, FpArgs{P.d_img, imgT, q, P.d_sino, G.d_rowinv, P.batch});
     B.bp = bp_params(&G, BpArgs{q, P.d_img, imgT, G.d_colinv, P.batch}, BATCH_PPT);
     B.batch = P.batch;
-    B.fp_bx = (G.R + FP_BLOCK * BATCH_RPT - 1) / (FP_BLOCK * BATCH_RPT);
+    B.fp_wb = (G.R + 64 * rpt - 1) / (64 * rpt);
     B.bp_tx = (G.N + BP_TW - 1) / BP_TW;
     B.bp_ty = (G.N + 16 * BATCH_PPT - 1) / (16 * BATCH_PPT);
-    B.fp_begin = plan.fp_total;
-    B.bp_begin = plan.bp_total;
-    plan.fp_total += (long)P.batch * G.A * B.fp_bx;
-    plan.bp_total += (long)P.batch * B.bp_tx * B.bp_ty;
+    B.fp_begin = args.fp_total;
+    B.bp_begin = args.bp_total;
+    args.fp_total += (long)P.batch * G.A * B.fp_wb;
+    args.bp_total += (long)P.batch * B.bp_tx * B.bp_ty;
     plan.Wmax = std::max(plan.Wmax, B.bp.W);
     plan.dr1 = plan.dr1 && G.drho == 1.0f && G.ncand == 2;
-    plan.host.push_back(B);
+    args.p[oi] = B;
     rc = launch_transpose(c, P.d_img, imgT, G.N, P.batch, s);
     if (rc) return rc;
   }
-  if (plan.fp_total >= (1L << 32) || plan.bp_total >= (1L << 32))
+  if (args.bp_total >= (1L << 32))
     return fail(c, SK_EINVAL, "problem set too large for one work queue");
   if (sizeof(float2) * BP_LB * (size_t)plan.Wmax > 60 * 1024)
     return fail(c, SK_EINVAL, "backprojection window exceeds LDS");
-  SK_HIP(c, hipMemcpyAsync(plan.d_probs, plan.host.data(), sizeof(BatchProb) * count,
-                           hipMemcpyHostToDevice, s));
-  SK_HIP(c, hipStreamSynchronize(s));  // host descriptors are read before they go out of scope
 
   const bool ex = c->exact != 0;
   const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
   int fp_grid = 0, bp_grid = 0;
-  const void* kfp = ex ? (const void*)k_fp_persist<true, BATCH_RPT> : (const void*)k_fp_persist<false, BATCH_RPT>;
+  const void* kfp = ex ? (const void*)k_fp_persist<true, 2, true> : (const void*)k_fp_persist<false, 2, true>;
   const void* kbp =
       plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, BATCH_PPT> : (const void*)k_bp_persist<false, 2, true, BATCH_PPT>)
                : (ex ? (const void*)k_bp_persist<true, 0, false, BATCH_PPT> : (const void*)k_bp_persist<false, 0, false, BATCH_PPT>);
   rc = persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
   if (!rc) rc = persistent_grid(c, kbp, 256, lds, &bp_grid);
   if (rc) return rc;
+  if (getenv("SYNKRO_VERBOSE"))
+    fprintf(stderr, "sk_sirt_batched: %d groups, fp items %ld (grid %d), bp items %ld (grid %d), W %d\n",
+            count, args.fp_total, fp_grid, args.bp_total, bp_grid, plan.Wmax);
 
   // Capture reset + iters x (fp, bp) once, launch once.
   hipStream_t cs = c->capture_stream;

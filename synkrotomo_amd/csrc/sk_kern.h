@@ -141,15 +141,14 @@ struct FpParams {
   const float* rowinv;  // FP_SIRT: [A][R]
 };
 
-// One work item: line j, bins [bx*FP_BLOCK*RPT, (bx+1)*FP_BLOCK*RPT), image bz.  Each wave
-// owns 64*RPT consecutive bins (lane l: bins first + l + 64k), all lanes stepping the same
-// frame row.  WEIGHTS == W_ADJ walks the same rays with the backprojector's lengths (B^T).
+// One wave's work item: line j, bins [wb*64*RPT, (wb+1)*64*RPT) of image bz (lane l: bins
+// first + l + 64k), all lanes stepping the same frame row.  WEIGHTS == W_ADJ walks the same
+// rays with the backprojector's lengths (B^T).  No barriers: waves are independent.
 template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
-__device__ __forceinline__ void fp_item(const FpParams& P, int j, int bx, int bz) {
+__device__ __forceinline__ void fp_item(const FpParams& P, int j, int wb, int bz) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int first = (bx * FP_WAVES + wave) * 64 * RPT;
-  if (first >= P.R) return;  // whole wave past the last bin (no barriers follow)
+  const int first = wb * 64 * RPT;
+  if (first >= P.R) return;  // whole wave past the last bin
   const int last = min(first + 64 * RPT - 1, P.R - 1);
   const Line L = P.lines[j];
   const int N = P.N, h = N >> 1;

@@ -46,7 +46,8 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
 // Grid: (bin blocks, lines, images).
 template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
 __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
-  fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.y, blockIdx.x, blockIdx.z);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.y, blockIdx.x * FP_WAVES + wave, blockIdx.z);
 }
 
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
