@@ -144,6 +144,34 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
 /* Apply previously tuned variants (-1 = the context default). */
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant);
 
+/* ---------------------------------------------------------------- learned correction */
+/* No reference counterpart.  Kernels of the correction network interleaved with SIRT:
+ * y = x + conv4(relu(conv3(relu(conv2(relu(conv1(x))))))), 3x3 convolutions, stride 1, zero
+ * padding 1.  Images: FP32 [B][H][W].  Activations: BF16 NHWC [B][H][W][32].  Weights: BF16
+ * [Cout][3][3][Cin].  Biases and all accumulation: FP32. */
+/* 32 -> 32 on MFMA.  bias != NULL: y = relu(conv(x) + bias).  bias == NULL: y = conv(x) (the
+ * data gradient, with flipped/transposed weights); mask != NULL zeroes x where mask <= 0. */
+int sk_conv32(sk_ctx* ctx, int B, int H, int W, const void* x, const void* mask, const void* w,
+              const float* bias, void* y, void* stream);
+/* 1 -> 32: h = relu(conv(x) + bias), w [32][3][3]. */
+int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, const float* bias,
+               void* h, void* stream);
+/* 32 -> 1 with residual: y = res + conv(h) + bias[0], w [3][3][32]. */
+int sk_conv_out(sk_ctx* ctx, int B, int H, int W, const void* h, const void* w, const float* bias,
+                const float* res, float* y, void* stream);
+/* Data gradients of sk_conv_out (dh from dy) and of sk_conv_in (dx from dh masked by h > 0). */
+int sk_conv_out_dgrad(sk_ctx* ctx, int B, int H, int W, const float* g, const void* w, void* dh,
+                      void* stream);
+int sk_conv_in_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
+                     const void* w, float* dx, void* stream);
+/* Weight and bias gradients (FP32, deterministic): kind 0 = conv1, 1 = conv2/conv3,
+ * 2 = conv4.  out = Cout*9*Cin weight gradients then Cout bias gradients. */
+int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, const void* mask,
+                  const void* x, float* out, void* stream);
+/* Random-ellipse phantoms: params [B][E][7] = (cx, cy, a, b, cos t, sin t, value) in FP64 on
+ * [-1,1]^2 pixel centres, sum clipped to [0,1] -> d_out [B][N][N] FP32. */
+int sk_phantoms(sk_ctx* ctx, int B, int N, int E, const double* d_params, float* d_out, void* stream);
+
 /* ---------------------------------------------------------------- sparse path */
 /* forwardprojection_sparse.main / backprojection_sparse.main
  * (futhark/forwardprojection_sparse.fut:26-40, futhark/backprojection_sparse.fut:26-37):

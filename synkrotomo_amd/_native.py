@@ -47,6 +47,13 @@ SIGNATURES = {
     "sk_sirt_batched": (_i, [_vp, _vp, _i, _i, _vp]),
     "sk_autotune": (_i, [_vp, _vp, _ip, _ip, _vp]),
     "sk_geom_set_variants": (_i, [_vp, _i, _i]),
+    "sk_conv32": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_in": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_out": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_out_dgrad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "sk_conv_in_dgrad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_wgrad": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sk_phantoms": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "sk_sparse_build": (_i, [_vp, _vp, _i, ctypes.POINTER(_vp), _vp]),
     "sk_sparse_count": (_i, [_vp, _vp, _i, _i64p, _vp]),
     "sk_mat_info": (_i, [_vp, _i64p, _i64p, _i64p]),

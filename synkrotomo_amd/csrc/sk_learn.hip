@@ -1,0 +1,462 @@
+// sk_learn.hip -- kernels of the learned SIRT correction (no reference counterpart) and the
+// GPU phantom generator used to train it.
+//
+// Network: y = x + conv4(relu(conv3(relu(conv2(relu(conv1(x))))))), 3x3 convolutions with
+// stride 1 and zero padding 1; conv1: 1 -> 32 channels, conv2/conv3: 32 -> 32, conv4: 32 -> 1.
+// Activations are BF16 in NHWC layout ([image][row][col][32]); images, biases and every
+// accumulation are FP32; weights are BF16 in [Cout][ky][kx][Cin] layout.
+//
+//   k_conv32      32 -> 32 (the FLOP-heavy layers) as an implicit GEMM on gfx950 MFMA
+//                 (v_mfma_f32_16x16x32_bf16): M = 16 output pixels of one row, N = 16 output
+//                 channels, K = 32 input channels of one tap; 9 taps accumulate.  A block
+//                 computes a 16x16-pixel tile x 32 channels; the 18x18x32 input patch and the
+//                 32x9x32 weights are staged in LDS.  The same kernel computes the data
+//                 gradient (weights flipped and transposed on the host side), with the ReLU
+//                 derivative of the layer's output fused into the staging of its input.
+//   k_conv_in/out 1 -> 32 and 32 -> 1 layers (288 MACs per pixel) on VALU, fused bias, ReLU
+//                 and residual; their data gradients likewise.
+//   k_wgrad       weight and bias gradients: per-block partial sums over pixel tiles
+//                 (deterministic), then k_reduce_partials.
+//   k_phantom     random-ellipse phantoms, evaluated in FP64 exactly as the numpy
+//                 generator (synkrotomo_amd/phantom.py) does.
+#include <algorithm>
+#include <type_traits>
+
+#include "sk_common.h"
+
+namespace sk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int CT = 16;    // conv tile side (pixels)
+constexpr int CP = 40;    // LDS channel stride (bf16 elements): 80 B, keeps 16 B alignment
+constexpr int NCH = 32;   // hidden channels
+
+// ---------------------------------------------------------------------- 32 -> 32, MFMA
+template <bool MASK_IN, bool BIAS_RELU>
+__global__ __launch_bounds__(256) void k_conv32(int H, int W, const __bf16* __restrict__ x,
+                                                const __bf16* __restrict__ mask,
+                                                const __bf16* __restrict__ w,
+                                                const float* __restrict__ bias,
+                                                __bf16* __restrict__ y) {
+  __shared__ __attribute__((aligned(16))) __bf16 sx[(CT + 2) * (CT + 2) * CP];
+  __shared__ __attribute__((aligned(16))) __bf16 sw[NCH * 9 * CP];
+  const int tiles_x = (W + CT - 1) / CT;
+  const int tx0 = (blockIdx.x % tiles_x) * CT, ty0 = (blockIdx.x / tiles_x) * CT;
+  const size_t img = (size_t)blockIdx.y * H * W;
+  for (int c = threadIdx.x; c < NCH * 9 * 4; c += 256) {
+    const int row = c >> 2, q = c & 3;  // row = co*9 + tap
+    *(bf16x8*)(sw + row * CP + q * 8) = *(const bf16x8*)(w + row * NCH + q * 8);
+  }
+  for (int c = threadIdx.x; c < (CT + 2) * (CT + 2) * 4; c += 256) {
+    const int pix = c >> 2, q = c & 3;
+    const int py = pix / (CT + 2), px = pix - py * (CT + 2);
+    const int gy = ty0 + py - 1, gx = tx0 + px - 1;
+    bf16x8 v;
+    for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
+    if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+      const size_t off = (img + (size_t)gy * W + gx) * NCH + q * 8;
+      v = *(const bf16x8*)(x + off);
+      if constexpr (MASK_IN) {
+        const bf16x8 m = *(const bf16x8*)(mask + off);
+        for (int i = 0; i < 8; ++i)
+          if (!((float)m[i] > 0.0f)) v[i] = (__bf16)0.0f;
+      }
+    }
+    *(bf16x8*)(sx + pix * CP + q * 8) = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  f32x4 acc[4][2];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int ky = t / 3, kx = t - 3 * (t / 3);
+    const bf16x8 b0 = *(const bf16x8*)(sw + ((0 * 16 + m) * 9 + t) * CP + 8 * kq);
+    const bf16x8 b1 = *(const bf16x8*)(sw + ((1 * 16 + m) * 9 + t) * CP + 8 * kq);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const int oy = wv * 4 + mb;
+      const bf16x8 a = *(const bf16x8*)(sx + ((oy + ky) * (CT + 2) + m + kx) * CP + 8 * kq);
+      acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[mb][0], 0, 0, 0);
+      acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[mb][1], 0, 0, 0);
+    }
+  }
+  // C layout: lane holds rows (pixels) 4*kq + i, column (channel) m of each 16x16 block.
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    const int oy = ty0 + wv * 4 + mb;
+    if (oy >= H) continue;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      const int co = nb * 16 + m;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ox = tx0 + 4 * kq + i;
+        if (ox >= W) continue;
+        float v = acc[mb][nb][i];
+        if constexpr (BIAS_RELU) v = fmaxf(v + bias[co], 0.0f);
+        y[(img + (size_t)oy * W + ox) * NCH + co] = (__bf16)v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- 1 -> 32, VALU
+// h[p][co] = relu(b[co] + sum_t w[co][t] x[p + off_t]); w: [32][9] bf16
+__global__ __launch_bounds__(256) void k_conv_in(int H, int W, long npix, const float* __restrict__ x,
+                                                 const __bf16* __restrict__ w,
+                                                 const float* __restrict__ bias,
+                                                 __bf16* __restrict__ h) {
+  __shared__ float sw[NCH * 9];
+  __shared__ float sb[NCH];
+  for (int i = threadIdx.x; i < NCH * 9; i += 256) sw[i] = (float)w[i];
+  if (threadIdx.x < NCH) sb[threadIdx.x] = bias[threadIdx.x];
+  __syncthreads();
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long b = p / ((long)H * W);
+  const int r = (int)(p - b * H * W);
+  const int py = r / W, px = r - py * W;
+  float v[9];
+  for (int t = 0; t < 9; ++t) {
+    const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+    v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[(b * H + yy) * W + xx] : 0.0f;
+  }
+  __bf16* out = h + p * NCH;
+  for (int q = 0; q < 4; ++q) {
+    bf16x8 o;
+    for (int i = 0; i < 8; ++i) {
+      const int co = q * 8 + i;
+      float s = sb[co];
+      for (int t = 0; t < 9; ++t) s = fmaf(sw[co * 9 + t], v[t], s);
+      o[i] = (__bf16)fmaxf(s, 0.0f);
+    }
+    *(bf16x8*)(out + q * 8) = o;
+  }
+}
+
+// ---------------------------------------------------------------------- 32 -> 1, VALU
+// y[p] = res[p] + b + sum_{t,ci} w[t][ci] h[p + off_t][ci]; w: [9][32] bf16
+__global__ __launch_bounds__(256) void k_conv_out(int H, int W, long npix, const __bf16* __restrict__ h,
+                                                  const __bf16* __restrict__ w,
+                                                  const float* __restrict__ bias,
+                                                  const float* __restrict__ res,
+                                                  float* __restrict__ y) {
+  __shared__ float sw[9 * NCH];
+  for (int i = threadIdx.x; i < 9 * NCH; i += 256) sw[i] = (float)w[i];
+  __syncthreads();
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long b = p / ((long)H * W);
+  const int r = (int)(p - b * H * W);
+  const int py = r / W, px = r - py * W;
+  float s = 0.0f;
+  for (int t = 0; t < 9; ++t) {
+    const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+    if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
+    const __bf16* hp = h + ((b * H + yy) * W + xx) * NCH;
+    for (int q = 0; q < 4; ++q) {
+      const bf16x8 v = *(const bf16x8*)(hp + q * 8);
+      for (int i = 0; i < 8; ++i) s = fmaf(sw[t * NCH + q * 8 + i], (float)v[i], s);
+    }
+  }
+  y[p] = res[p] + (s + bias[0]);
+}
+
+// d h[q][ci] = sum_t w[t][ci] g[q - off_t]   (data gradient of k_conv_out)
+__global__ __launch_bounds__(256) void k_conv_out_dgrad(int H, int W, long npix, const float* __restrict__ g,
+                                                        const __bf16* __restrict__ w,
+                                                        __bf16* __restrict__ dh) {
+  __shared__ float sw[9 * NCH];
+  for (int i = threadIdx.x; i < 9 * NCH; i += 256) sw[i] = (float)w[i];
+  __syncthreads();
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long b = p / ((long)H * W);
+  const int r = (int)(p - b * H * W);
+  const int py = r / W, px = r - py * W;
+  float gv[9];
+  for (int t = 0; t < 9; ++t) {
+    const int yy = py - (t / 3 - 1), xx = px - (t % 3 - 1);
+    gv[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? g[(b * H + yy) * W + xx] : 0.0f;
+  }
+  __bf16* out = dh + p * NCH;
+  for (int q = 0; q < 4; ++q) {
+    bf16x8 o;
+    for (int i = 0; i < 8; ++i) {
+      const int ci = q * 8 + i;
+      float s = 0.0f;
+      for (int t = 0; t < 9; ++t) s = fmaf(sw[t * NCH + ci], gv[t], s);
+      o[i] = (__bf16)s;
+    }
+    *(bf16x8*)(out + q * 8) = o;
+  }
+}
+
+// dx[q] = sum_{t,co} w[co][t] g1[q - off_t][co],  g1 = dh * (h > 0)   (data gradient of k_conv_in)
+__global__ __launch_bounds__(256) void k_conv_in_dgrad(int H, int W, long npix, const __bf16* __restrict__ dh,
+                                                       const __bf16* __restrict__ hmask,
+                                                       const __bf16* __restrict__ w,
+                                                       float* __restrict__ dx) {
+  __shared__ float sw[NCH * 9];
+  for (int i = threadIdx.x; i < NCH * 9; i += 256) sw[i] = (float)w[i];
+  __syncthreads();
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long b = p / ((long)H * W);
+  const int r = (int)(p - b * H * W);
+  const int py = r / W, px = r - py * W;
+  float s = 0.0f;
+  for (int t = 0; t < 9; ++t) {
+    const int yy = py - (t / 3 - 1), xx = px - (t % 3 - 1);
+    if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
+    const size_t off = ((b * H + yy) * W + xx) * NCH;
+    for (int q = 0; q < 4; ++q) {
+      const bf16x8 gv = *(const bf16x8*)(dh + off + q * 8);
+      const bf16x8 mv = *(const bf16x8*)(hmask + off + q * 8);
+      for (int i = 0; i < 8; ++i)
+        if ((float)mv[i] > 0.0f) s = fmaf(sw[(q * 8 + i) * 9 + t], (float)gv[i], s);
+    }
+  }
+  dx[p] = s;
+}
+
+// ---------------------------------------------------------------------- weight gradients
+// dW[co][t][ci] = sum_p G[p][co] X[p + off_t][ci],  db[co] = sum_p G[p][co]
+// G: CO channels (bf16 NHWC, optionally masked by M > 0) or an FP32 image (CO == 1).
+// X: CI channels (bf16 NHWC) or an FP32 image (CI == 1).  Each block accumulates over a
+// grid-stride set of 16x16 tiles and writes CO*9*CI + CO partial sums.
+template <int CO, int CI>
+__global__ __launch_bounds__(256) void k_wgrad(int H, int W, int B, const void* __restrict__ Gp,
+                                               const __bf16* __restrict__ M,
+                                               const void* __restrict__ Xp,
+                                               float* __restrict__ partials) {
+  constexpr int NOUT = CO * 9 * CI + CO;
+  constexpr int PER = (NOUT + 255) / 256;
+  // 32-channel operands are BF16 already and stay BF16 in LDS; FP32 images stay FP32.
+  using GT = typename std::conditional<CO == 1, float, __bf16>::type;
+  using XT = typename std::conditional<CI == 1, float, __bf16>::type;
+  __shared__ GT sg[CT * CT * CO];
+  __shared__ XT sx[(CT + 2) * (CT + 2) * CI];
+  float acc[PER];
+  for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
+  const int tiles_x = (W + CT - 1) / CT, tiles_y = (H + CT - 1) / CT;
+  const long ntiles = (long)tiles_x * tiles_y * B;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int b = (int)(tile / ((long)tiles_x * tiles_y));
+    const int tr = (int)(tile - (long)b * tiles_x * tiles_y);
+    const int ty0 = (tr / tiles_x) * CT, tx0 = (tr % tiles_x) * CT;
+    for (int e = threadIdx.x; e < CT * CT * CO; e += 256) {
+      const int pix = e / CO, co = e - pix * CO;
+      const int gy = ty0 + pix / CT, gx = tx0 + pix % CT;
+      float v = 0.0f;
+      if (gy < H && gx < W) {
+        const size_t p = ((size_t)b * H + gy) * W + gx;
+        if constexpr (CO == 1) {
+          v = ((const float*)Gp)[p];
+        } else {
+          v = (float)((const __bf16*)Gp)[p * CO + co];
+          if (M && !((float)M[p * CO + co] > 0.0f)) v = 0.0f;
+        }
+      }
+      sg[e] = (GT)v;
+    }
+    for (int e = threadIdx.x; e < (CT + 2) * (CT + 2) * CI; e += 256) {
+      const int pix = e / CI, ci = e - pix * CI;
+      const int py = pix / (CT + 2), px = pix % (CT + 2);
+      const int gy = ty0 + py - 1, gx = tx0 + px - 1;
+      float v = 0.0f;
+      if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+        const size_t p = ((size_t)b * H + gy) * W + gx;
+        if constexpr (CI == 1)
+          v = ((const float*)Xp)[p];
+        else
+          v = (float)((const __bf16*)Xp)[p * CI + ci];
+      }
+      sx[e] = (XT)v;
+    }
+    __syncthreads();
+    for (int k = 0; k < PER; ++k) {
+      const int o = threadIdx.x + 256 * k;
+      if (o >= NOUT) break;
+      float s = 0.0f;
+      if (o < CO * 9 * CI) {
+        const int co = o / (9 * CI), rem = o - co * 9 * CI, t = rem / CI, ci = rem - t * CI;
+        const int ky = t / 3, kx = t % 3;
+        for (int py = 0; py < CT; ++py)
+          for (int px = 0; px < CT; ++px)
+            s = fmaf((float)sg[(py * CT + px) * CO + co],
+                     (float)sx[((py + ky) * (CT + 2) + px + kx) * CI + ci], s);
+      } else {
+        const int co = o - CO * 9 * CI;
+        for (int pix = 0; pix < CT * CT; ++pix) s += (float)sg[pix * CO + co];
+      }
+      acc[k] += s;
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < PER; ++k) {
+    const int o = threadIdx.x + 256 * k;
+    if (o < NOUT) partials[(size_t)blockIdx.x * NOUT + o] = acc[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partials, int nblocks,
+                                                         int nout, float* __restrict__ out) {
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  if (o >= nout) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * nout + o];
+  out[o] = s;
+}
+
+// ---------------------------------------------------------------------- phantoms
+// params[b][e] = (cx, cy, a, b, cos t, sin t, value); pixel centres at (2j+1)/N - 1.
+__global__ __launch_bounds__(256) void k_phantom(int N, int E, long npix, const double* __restrict__ params,
+                                                 float* __restrict__ out) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  const long b = p / ((long)N * N);
+  const int r = (int)(p - b * N * N);
+  const int i = r / N, j = r - i * N;
+  const double x = (2.0 * j + 1.0) / N - 1.0, y = (2.0 * i + 1.0) / N - 1.0;
+  double acc = 0.0;
+  const double* P = params + b * E * 7;
+  for (int e = 0; e < E; ++e) {
+    const double* q = P + e * 7;
+    const double dx = x - q[0], dy = y - q[1];
+    const double u = (dx * q[4] + dy * q[5]) / q[2];
+    const double w = (-dx * q[5] + dy * q[4]) / q[3];
+    acc += (u * u + w * w <= 1.0) ? q[6] : 0.0;
+  }
+  acc = acc < 0.0 ? 0.0 : (acc > 1.0 ? 1.0 : acc);
+  out[p] = (float)acc;
+}
+
+}  // namespace sk
+
+using namespace sk;
+
+static int check_ctx(sk_ctx* ctx) {
+  if (!ctx) return SK_EINVAL;
+  SK_HIP(&ctx->c, hipSetDevice(ctx->c.device));
+  return SK_OK;
+}
+
+extern "C" {
+
+int sk_conv32(sk_ctx* ctx, int B, int H, int W, const void* x, const void* mask, const void* w,
+              const float* bias, void* y, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || B > 65535 || H < 1 || W < 1 || !x || !w || !y)
+    return fail(&ctx->c, SK_EINVAL, "conv32: bad arguments");
+  const bool relu = bias != nullptr;
+  dim3 grid(((W + CT - 1) / CT) * ((H + CT - 1) / CT), B);
+  hipStream_t s = (hipStream_t)stream;
+  const __bf16 *X = (const __bf16*)x, *Mk = (const __bf16*)mask, *Wt = (const __bf16*)w;
+  __bf16* Y = (__bf16*)y;
+  if (mask && relu)
+    hipLaunchKernelGGL((k_conv32<true, true>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+  else if (mask)
+    hipLaunchKernelGGL((k_conv32<true, false>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+  else if (relu)
+    hipLaunchKernelGGL((k_conv32<false, true>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+  else
+    hipLaunchKernelGGL((k_conv32<false, false>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+  return launched(&ctx->c, "conv32 kernel");
+}
+
+int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, const float* bias,
+               void* h, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || H < 1 || W < 1 || !x || !w || !bias || !h) return fail(&ctx->c, SK_EINVAL, "conv_in: bad arguments");
+  const long npix = (long)B * H * W;
+  hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream, H, W,
+                     npix, x, (const __bf16*)w, bias, (__bf16*)h);
+  return launched(&ctx->c, "conv_in kernel");
+}
+
+int sk_conv_out(sk_ctx* ctx, int B, int H, int W, const void* h, const void* w, const float* bias,
+                const float* res, float* y, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || H < 1 || W < 1 || !h || !w || !bias || !res || !y)
+    return fail(&ctx->c, SK_EINVAL, "conv_out: bad arguments");
+  const long npix = (long)B * H * W;
+  hipLaunchKernelGGL(k_conv_out, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream, H, W,
+                     npix, (const __bf16*)h, (const __bf16*)w, bias, res, y);
+  return launched(&ctx->c, "conv_out kernel");
+}
+
+int sk_conv_out_dgrad(sk_ctx* ctx, int B, int H, int W, const float* g, const void* w, void* dh,
+                      void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || H < 1 || W < 1 || !g || !w || !dh) return fail(&ctx->c, SK_EINVAL, "conv_out_dgrad: bad arguments");
+  const long npix = (long)B * H * W;
+  hipLaunchKernelGGL(k_conv_out_dgrad, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     H, W, npix, g, (const __bf16*)w, (__bf16*)dh);
+  return launched(&ctx->c, "conv_out_dgrad kernel");
+}
+
+int sk_conv_in_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
+                     const void* w, float* dx, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || H < 1 || W < 1 || !dh || !hmask || !w || !dx)
+    return fail(&ctx->c, SK_EINVAL, "conv_in_dgrad: bad arguments");
+  const long npix = (long)B * H * W;
+  hipLaunchKernelGGL(k_conv_in_dgrad, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     H, W, npix, (const __bf16*)dh, (const __bf16*)hmask, (const __bf16*)w, dx);
+  return launched(&ctx->c, "conv_in_dgrad kernel");
+}
+
+// kind 0: conv1 (G = 32-ch gradient masked by h1, X = fp32 image) -> dW [32][9], db [32]
+// kind 1: conv2/3 (G masked, X = 32-ch activation)                  -> dW [32][9][32], db [32]
+// kind 2: conv4 (G = fp32 image gradient, X = 32-ch activation)     -> dW [9][32], db [1]
+// out: CO*9*CI weight gradients followed by CO bias gradients (fp32).
+int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, const void* mask,
+                  const void* x, float* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx* c = &ctx->c;
+  if (B < 1 || H < 1 || W < 1 || !g || !x || !out || kind < 0 || kind > 2)
+    return fail(c, SK_EINVAL, "conv_wgrad: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const long ntiles = (long)((W + CT - 1) / CT) * ((H + CT - 1) / CT) * B;
+  const int nblocks = (int)std::min<long>(ntiles, 1024);
+  const int nout = kind == 0 ? NCH * 9 + NCH : (kind == 1 ? NCH * 9 * NCH + NCH : 9 * NCH + 1);
+  void* ws = nullptr;
+  rc = scratch(c, sizeof(float) * (size_t)nblocks * nout, s, &ws);
+  if (rc) return rc;
+  float* part = (float*)ws;
+  const __bf16* Mk = (const __bf16*)mask;
+  if (kind == 0)
+    hipLaunchKernelGGL((k_wgrad<NCH, 1>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
+  else if (kind == 1)
+    hipLaunchKernelGGL((k_wgrad<NCH, NCH>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
+  else
+    hipLaunchKernelGGL((k_wgrad<1, NCH>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
+  rc = launched(c, "wgrad kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nout + 255) / 256), dim3(256), 0, s, part, nblocks, nout, out);
+  return launched(c, "partials reduction kernel");
+}
+
+int sk_phantoms(sk_ctx* ctx, int B, int N, int E, const double* d_params, float* d_out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (B < 1 || N < 1 || E < 0 || !d_params || !d_out) return fail(&ctx->c, SK_EINVAL, "phantoms: bad arguments");
+  const long npix = (long)B * N * N;
+  hipLaunchKernelGGL(k_phantom, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream, N, E,
+                     npix, d_params, d_out);
+  return launched(&ctx->c, "phantom kernel");
+}
+
+}  // extern "C"

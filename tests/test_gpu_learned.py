@@ -1,0 +1,134 @@
+"""The correction network's kernels against a CPU FP32 reference (torch conv2d on the same
+BF16-rounded operands), its gradients, the GPU phantom generator, and the learned-SIRT
+fallback rule."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).float()
+
+
+def _reference(x, net):
+    """CPU FP32 forward with the kernels' rounding points: BF16 weights and activations."""
+    w1 = _bf(net.w1.detach().cpu()).unsqueeze(1)                      # [32,1,3,3]
+    w2 = _bf(net.w2.detach().cpu()).permute(0, 3, 1, 2)               # [32,32,3,3]
+    w3 = _bf(net.w3.detach().cpu()).permute(0, 3, 1, 2)
+    w4 = _bf(net.w4.detach().cpu()).permute(2, 0, 1).unsqueeze(0)     # [1,32,3,3]
+    xc = x.detach().cpu().unsqueeze(1)
+    h1 = _bf(F.relu(F.conv2d(xc, w1, net.b1.detach().cpu(), padding=1)))
+    h2 = _bf(F.relu(F.conv2d(h1, w2, net.b2.detach().cpu(), padding=1)))
+    h3 = _bf(F.relu(F.conv2d(h2, w3, net.b3.detach().cpu(), padding=1)))
+    return (xc + F.conv2d(h3, w4, net.b4.detach().cpu(), padding=1)).squeeze(1)
+
+
+def _net(ctx, seed=1):
+    from synkrotomo_amd.learned import Correction
+    net = Correction(ctx, seed=seed)
+    with torch.no_grad():
+        net.w4.copy_(torch.randn(net.w4.shape, generator=torch.Generator().manual_seed(9)).cuda() * 0.05)
+        for b in (net.b1, net.b2, net.b3, net.b4):
+            b.copy_(torch.randn(b.shape, generator=torch.Generator().manual_seed(4)).cuda() * 0.1)
+    return net
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 16, 16), (2, 37, 23), (3, 64, 48)])
+def test_forward_matches_cpu_fp32_reference(ctx, B, H, W):
+    net = _net(ctx)
+    x = torch.rand(B, H, W, device="cuda")
+    y = net(x).detach()
+    ref = _reference(x, net)
+    assert rel_l2(y.cpu().numpy(), ref.numpy()) < 1e-3
+
+
+def _layer_grads(fn, inputs, g):
+    """FP64 autograd of one layer; returns the gradients of every input."""
+    ins = [t.detach().double().clone().requires_grad_(True) for t in inputs]
+    out = fn(*ins)
+    return torch.autograd.grad(out, ins, g.double())
+
+
+def test_backward_matches_cpu_reference(ctx):
+    """Gradients w.r.t. the input and every weight against FP64 autograd of each layer, with
+    the kernels' BF16 rounding points (weights, activations, activation gradients)."""
+    net = _net(ctx, seed=2)
+    x = torch.rand(2, 40, 36, device="cuda", requires_grad=True)
+    gy = torch.randn(2, 40, 36, device="cuda")
+    y = net(x)
+    grads = torch.autograd.grad(y, [x, net.w1, net.b1, net.w2, net.b2, net.w3, net.b3, net.w4, net.b4], gy)
+    cpu = lambda t: t.detach().cpu()
+    w1, w2, w3, w4 = (_bf(cpu(w)) for w in (net.w1, net.w2, net.w3, net.w4))
+    b1, b2, b3, b4 = (cpu(b) for b in (net.b1, net.b2, net.b3, net.b4))
+    xc = cpu(x).unsqueeze(1)
+    c1 = lambda inp, w, b: F.relu(F.conv2d(inp, w.unsqueeze(1).contiguous(), b, padding=1))
+    c32 = lambda inp, w, b: F.relu(F.conv2d(inp, w.permute(0, 3, 1, 2).contiguous(), b, padding=1))
+    c4 = lambda inp, w, b: F.conv2d(inp, w.permute(2, 0, 1).unsqueeze(0).contiguous(), b, padding=1)
+    h1 = _bf(c1(xc, w1, b1))
+    h2 = _bf(c32(h1, w2, b2))
+    h3 = _bf(c32(h2, w3, b3))
+    g = cpu(gy).unsqueeze(1)
+    dh3, gw4, gb4 = _layer_grads(c4, [h3, w4, b4], g)
+    dh2, gw3, gb3 = _layer_grads(c32, [h2, w3, b3], _bf(dh3.float()))
+    dh1, gw2, gb2 = _layer_grads(c32, [h1, w2, b2], _bf(dh2.float()))
+    dx, gw1, gb1 = _layer_grads(c1, [xc, w1, b1], _bf(dh1.float()))
+    dx = dx + g.double()
+    ref = [dx.squeeze(1), gw1, gb1, gw2, gb2, gw3, gb3, gw4, gb4]
+    names = ["x", "w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4"]
+    for n, gg, r in zip(names, grads, ref):
+        err = rel_l2(gg.cpu().numpy(), r.numpy())
+        assert err < 2e-3, (n, err)
+
+
+def test_conv32_is_mfma_gemm_exactly_on_bf16_inputs(ctx):
+    """With BF16-exact operands and sums the MFMA conv equals an FP64 convolution exactly."""
+    from synkrotomo_amd import _native as nat
+    from synkrotomo_amd.projector import _ptr, _stream
+    B, H, W = 1, 20, 20
+    x = torch.randint(-2, 3, (B, H, W, 32), device="cuda").to(torch.bfloat16)
+    w = torch.randint(-1, 2, (32, 3, 3, 32), device="cuda").to(torch.bfloat16)
+    y = torch.empty_like(x)
+    nat.check(nat.lib().sk_conv32(ctx.handle, B, H, W, _ptr(x), None, _ptr(w), None, _ptr(y),
+                                  _stream(0)), ctx.handle)
+    ref = F.conv2d(x.double().cpu().permute(0, 3, 1, 2), w.double().cpu().permute(0, 3, 1, 2), padding=1)
+    np.testing.assert_array_equal(y.float().cpu().permute(0, 3, 1, 2).numpy(),
+                                  ref.to(torch.bfloat16).float().numpy())
+
+
+def test_gpu_phantoms_equal_numpy_generator(ctx):
+    from synkrotomo_amd.learned import gpu_phantoms
+    from synkrotomo_amd.phantom import ellipse_params, render
+    imgs = gpu_phantoms(ctx, np.random.default_rng(77), 3, 64, 10).cpu().numpy()
+    rng = np.random.default_rng(77)
+    for i in range(3):
+        np.testing.assert_array_equal(imgs[i], render(64, *ellipse_params(rng, 10)))
+
+
+def test_learned_sirt_fallback_on_broken_network(ctx):
+    """A network that returns NaN, or one that makes things worse, yields plain SIRT."""
+    from synkrotomo_amd.learned import Correction, LearnedSIRT, gpu_phantoms
+    from synkrotomo_amd.projector import centred_bins, uniform_angles
+    N, A = 64, 45
+    R, rz = centred_bins(N)
+    geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
+    x = gpu_phantoms(ctx, np.random.default_rng(5), 2, N)
+    b = geo.fp(x)
+    net = Correction(ctx)
+    ls = LearnedSIRT(geo, net, stages=3)
+    plain = torch.zeros(2, N, N, device="cuda")
+    geo.sirt_(plain, b, 3)
+    with torch.no_grad():
+        net.b4.fill_(float("nan"))
+    out, fell = ls.reconstruct(b)
+    assert bool(fell.all()) or torch.allclose(out, plain, atol=1e-5)
+    torch.testing.assert_close(out, plain, rtol=1e-5, atol=1e-5)
+    with torch.no_grad():
+        net.b4.fill_(5.0)  # a constant offset: residual far above plain SIRT's
+    out, fell = ls.reconstruct(b)
+    assert bool(fell.all())
+    torch.testing.assert_close(out, plain, rtol=1e-5, atol=1e-5)
