@@ -192,6 +192,9 @@ int sk_mat_arrays(const sk_mat* m, const int64_t** d_rowptr, const int32_t** d_c
 /* Copy the CSR arrays into caller-owned device buffers (rows+1, nnz, nnz entries). */
 int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols, float* d_vals,
                 void* stream);
+/* SpMV kernel: 0 = one wave per row, 1 = one lane per row (64 adjacent rows in lockstep);
+ * -1 (default) = time both on the first sk_spmv call and keep the faster. */
+int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 /* d_out[batch][rows] = M . d_in[batch][cols] */
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);
 void sk_mat_destroy(sk_mat* m);

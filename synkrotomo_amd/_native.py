@@ -59,6 +59,7 @@ SIGNATURES = {
     "sk_mat_info": (_i, [_vp, _i64p, _i64p, _i64p]),
     "sk_mat_arrays": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "sk_mat_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sk_mat_set_spmv_variant": (_i, [_vp, _i]),
     "sk_spmv": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_mat_destroy": (None, [_vp]),
 }

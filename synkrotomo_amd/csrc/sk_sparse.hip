@@ -22,6 +22,7 @@
 
 struct sk_mat {
   int kind = 0;
+  int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep); -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
@@ -279,6 +280,44 @@ static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr
   return SK_OK;
 }
 
+// Lockstep SpMV: one lane per row, the 64 rows of a wave advance through their entries
+// together.  Adjacent rows of A are rays of adjacent bins of one angle and adjacent rows
+// of B are neighbouring pixels, so at each step the wave's x gathers fall on neighbouring
+// addresses; each lane streams its own row's columns and values, summed in CSR order.
+__global__ __launch_bounds__(256) void k_spmv_lockstep(const int64_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const float* __restrict__ val, long rows,
+                                                       long ncols, const float* __restrict__ x,
+                                                       float* __restrict__ y) {
+  const long row = (long)blockIdx.x * 256 + threadIdx.x;
+  const float* xb = x + (size_t)blockIdx.y * ncols;
+  long beg = 0, end = 0;
+  if (row < rows) {
+    beg = rowptr[row];
+    end = rowptr[row + 1];
+  }
+  const int len = (int)(end - beg);
+  int maxlen = len;
+  for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, off));
+  float acc = 0.0f;
+  int k = 0;
+  for (; k + 4 <= maxlen; k += 4) {
+    float v[4];
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool ok = k + u < len;
+      c[u] = ok ? col[beg + k + u] : 0;
+      v[u] = ok ? val[beg + k + u] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = fmaf(v[u], xb[c[u]], acc);
+  }
+  for (; k < maxlen; ++k)
+    if (k < len) acc = fmaf(val[beg + k], xb[col[beg + k]], acc);
+  if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
 static void launch_rows(const Geom& G, int kind, bool fill, int32_t* counts, const int64_t* rowptr,
                         int32_t* cols, float* vals, hipStream_t s) {
   const long rows = kind == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
@@ -394,6 +433,12 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
   return rc;
 }
 
+int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
+  if (!m || variant < -1 || variant > 1) return SK_EINVAL;
+  m->spmv_variant = variant;
+  return SK_OK;
+}
+
 int sk_mat_info(const sk_mat* m, int64_t* rows, int64_t* cols, int64_t* nnz) {
   if (!m) return SK_EINVAL;
   if (rows) *rows = m->rows;
@@ -437,9 +482,35 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   if (m->device != c->device) return fail(c, SK_EINVAL, "matrix belongs to another device");
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
-  dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-  hipLaunchKernelGGL(k_spmv<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val, (long)m->rows,
-                     (long)m->cols, d_in, d_out);
+  auto run = [&](int v) {
+    if (v == 1) {
+      dim3 grid((unsigned)((m->rows + 255) / 256), batch);
+      hipLaunchKernelGGL(k_spmv_lockstep, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                         (long)m->rows, (long)m->cols, d_in, d_out);
+    } else {
+      dim3 grid((unsigned)((m->rows + 3) / 4), batch);
+      hipLaunchKernelGGL(k_spmv<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                         (long)m->rows, (long)m->cols, d_in, d_out);
+    }
+  };
+  if (m->spmv_variant < 0) {
+    // First use: time both kernels once on this matrix and keep the faster.
+    hipEvent_t e[3];
+    for (auto& ev : e) SK_HIP(c, hipEventCreate(&ev));
+    SK_HIP(c, hipEventRecord(e[0], s));
+    run(0);
+    SK_HIP(c, hipEventRecord(e[1], s));
+    run(1);
+    SK_HIP(c, hipEventRecord(e[2], s));
+    SK_HIP(c, hipEventSynchronize(e[2]));
+    float t0 = 0.0f, t1 = 0.0f;
+    SK_HIP(c, hipEventElapsedTime(&t0, e[0], e[1]));
+    SK_HIP(c, hipEventElapsedTime(&t1, e[1], e[2]));
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    const_cast<sk_mat*>(m)->spmv_variant = t1 < t0 ? 1 : 0;
+    return launched(c, "SpMV kernel");
+  }
+  run(m->spmv_variant);
   return launched(c, "SpMV kernel");
 }
 

@@ -192,17 +192,20 @@ class LearnedSIRT:
     def sirt_step(self, x, b):
         return _SirtStep.apply(x, b, self.geom, self.w.rowinv, self.w.colinv)
 
-    def unrolled(self, b: torch.Tensor, checkpoint: bool = True) -> torch.Tensor:
-        """Differentiable forward used for training: images [B, N, N]."""
+    def unrolled(self, b: torch.Tensor, checkpoint: bool = True, all_stages: bool = False):
+        """Differentiable forward used for training: images [B, N, N] after the last stage
+        (or the list of every stage's output)."""
         from torch.utils.checkpoint import checkpoint as ck
         B = b.shape[0]
         x = torch.zeros((B, self.geom.N, self.geom.N), device=b.device)
+        outs = []
         for _ in range(self.stages):
             if checkpoint and torch.is_grad_enabled():
                 x = ck(lambda xx, bb: self.net(self.sirt_step(xx, bb)), x, b, use_reentrant=False)
             else:
                 x = self.net(self.sirt_step(x, b))
-        return x
+            outs.append(x)
+        return outs if all_stages else x
 
     @torch.no_grad()
     def reconstruct(self, b: torch.Tensor):

@@ -306,6 +306,10 @@ class Matrix:
                                         _stream(self.geom.ctx.device)), self.geom.ctx.handle, "matrix copy")
         return rp, c[: self.nnz], v[: self.nnz]
 
+    def set_spmv_variant(self, variant: int) -> None:
+        """0: wave per row, 1: lane per row (lockstep), -1: time both on next use."""
+        nat.check(nat.lib().sk_mat_set_spmv_variant(self.handle, int(variant)), self.geom.ctx.handle)
+
     def spmv(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         g = self.geom
         batch = g._check(x, self.cols, "input")

@@ -50,10 +50,13 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
     x = rng.random(N * N).astype(np.float32)
     y = rng.random(ang.size * R).astype(np.float32)
     geo = ctx.geometry(ang, rz, dr, R, N)
-    ax = geo.sparse("fp").spmv(_dev(x)).cpu().numpy()
-    by = geo.sparse("bp").spmv(_dev(y)).cpu().numpy()
-    assert rel_l2(ax, oracle.fp(ang, rz, dr, R, x)) <= TOL
-    assert rel_l2(by, oracle.bp(ang, rz, dr, N, y)) <= TOL
+    for variant in (0, 1):
+        for kind in ("fp", "bp"):
+            geo.sparse(kind).set_spmv_variant(variant)
+        ax = geo.sparse("fp").spmv(_dev(x)).cpu().numpy()
+        by = geo.sparse("bp").spmv(_dev(y)).cpu().numpy()
+        assert rel_l2(ax, oracle.fp(ang, rz, dr, R, x)) <= TOL
+        assert rel_l2(by, oracle.bp(ang, rz, dr, N, y)) <= TOL
     geo.drop_sparse()
 
 
