@@ -125,6 +125,18 @@ def batched_leg(ctx, iters: int = 30) -> dict:
             "batched_gups": round(updates / t_batched / 1e9, 1), "identical_to_per_group": same}
 
 
+def learned_leg(ctx, n_test: int = 32) -> dict:
+    """Time to a data residual, learned SIRT (5 x (SIRT step + BF16 CNN), trained weights in
+    synkrotomo_amd/weights) vs plain SIRT, on held-out 512^2 phantoms, 360 angles, 725 bins,
+    Poisson noise I0 = 1e4.  Targets: the literal 1e-2 and 1.1x each phantom's noise floor
+    (the 1e-2 target lies below the noise floor at I0 = 1e4)."""
+    from synkrotomo_amd import learned_run
+    if not os.path.exists(learned_run.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights"}
+    return learned_run.evaluate(ctx, learned_run.DEFAULT_WEIGHTS, n_test, batch=16, max_iters=200,
+                                verbose=False)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +146,7 @@ def main() -> int:
     ap.add_argument("--exact", action="store_true", help="IEEE division everywhere (bitwise mode)")
     ap.add_argument("--no-sparse", action="store_true", help="skip the explicit-matrix leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-SIRT leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-SIRT leg")
     ap.add_argument("--batched-iters", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use default kernel variants")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -223,7 +236,10 @@ def main() -> int:
         "cpu_baseline": None,
         "sparse": None,
         "batched": None,
+        "learned": None,
     }
+    if not args.no_learned:
+        out["learned"] = learned_leg(ctx)
     if not args.no_batched:
         out["batched"] = batched_leg(ctx, args.batched_iters)
     if not args.no_sparse:

@@ -54,7 +54,8 @@ enum {
 /* Operators that can be made explicit by sk_sparse_build. */
 enum {
   SK_MAT_FP = 0, /* A: rows = rays (angle-major), columns = pixels */
-  SK_MAT_BP = 1  /* B: rows = pixels, columns = sinogram entries (the backprojector) */
+  SK_MAT_BP = 1, /* B: rows = pixels, columns = sinogram entries (the backprojector) */
+  SK_MAT_ADJ = 2 /* A^T (the CSC form of A): rows = pixels, columns = rays */
 };
 
 /* ---------------------------------------------------------------- context */
@@ -137,7 +138,7 @@ typedef struct {
  * whole sequence runs as one hipGraph.  Synchronizes `stream` before returning. */
 int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream);
 
-/* No reference counterpart: time the kernel variants (fp: 1 or 2 rays per lane; bp: 16x16,
+/* No reference counterpart: time the kernel variants (fp: 1, 2 or 4 rays per lane; bp: 16x16,
  * 16x32, 16x64 pixel tiles) on this geometry and keep the fastest for it.  Variants compute
  * identical results.  Synchronizes `stream`. */
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
@@ -180,10 +181,14 @@ int sk_phantoms(sk_ctx* ctx, int B, int N, int E, const double* d_params, float*
  *              the forward projector's lengths (bitwise equal to the FP32 restatement).
  *   SK_MAT_BP  B: rows = pixels, cols = sinogram entries; the backprojector's non-zero
  *              intersect_fact lengths, lines in steep-then-flat order.
+ *   SK_MAT_ADJ A^T, i.e. A in CSC form: rows = pixels, cols = rays; exactly A's entries of
+ *              each pixel (explicit zeros included), values bit for bit A's.  No reference
+ *              counterpart (the reference's backprojection_sparse is B); A^T.r is the exact
+ *              adjoint of the sparse forward projection.
  * 64-bit row pointers, 32-bit columns, f32 values, all owned by the matrix.  The call
  * synchronizes `stream` once (to size the column/value arrays). */
 int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stream);
-/* nnz of SK_MAT_FP / SK_MAT_BP without building the matrix (synchronizes `stream`). */
+/* nnz of an SK_MAT_* matrix without building it (synchronizes `stream`). */
 int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* stream);
 int sk_mat_info(const sk_mat* m, int64_t* rows, int64_t* cols, int64_t* nnz);
 /* Device pointers of the CSR arrays (valid until sk_mat_destroy). */
@@ -192,8 +197,9 @@ int sk_mat_arrays(const sk_mat* m, const int64_t** d_rowptr, const int32_t** d_c
 /* Copy the CSR arrays into caller-owned device buffers (rows+1, nnz, nnz entries). */
 int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols, float* d_vals,
                 void* stream);
-/* SpMV kernel: 0 = one wave per row, 1 = one lane per row (64 adjacent rows in lockstep);
- * -1 (default) = time both on the first sk_spmv call and keep the faster. */
+/* SpMV kernel: 0 = one wave per row, 1 = one lane per row (64 adjacent rows in lockstep),
+ * 2 = one wave per row with 16-byte column/value streams; -1 (default) = time all on the
+ * first sk_spmv call and keep the fastest. */
 int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 /* d_out[batch][rows] = M . d_in[batch][cols] */
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libsynkro.so")
 
 SK_OK, SK_EINVAL, SK_ETYPE, SK_EHIP, SK_ENOMEM = 0, 1, 2, 3, 4
 SK_OPT_EXACT, SK_OPT_FP_VARIANT, SK_OPT_BP_VARIANT = 1, 2, 3
-SK_MAT_FP, SK_MAT_BP = 0, 1
+SK_MAT_FP, SK_MAT_BP, SK_MAT_ADJ = 0, 1, 2
 
 _lib = None
 _lock = threading.Lock()

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchArgs a, unsi
 template <bool EXACT, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
-  __shared__ int w0s[BP_LB];
+  __shared__ BpLineLds lsh[BP_LB];
   __shared__ unsigned item_sh;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr[2] = 0u;  // the fp queue of the next iteration
   for (;;) {
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned*
     const long rem = local - (long)bz * per_img;
     const int ty = (int)(rem / B.bp_tx);
     const int tx = (int)(rem - (long)ty * B.bp_tx);
-    bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, u32(tx), u32(ty), u32(bz), win, w0s);
+    bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, u32(tx), u32(ty), u32(bz), win, lsh);
   }
 }
 
@@ -287,7 +287,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
 
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
   if (!g) return SK_EINVAL;
-  if (fp_variant < -1 || fp_variant > 1 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
+  if (fp_variant < -1 || fp_variant > 2 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
   g->g.fp_variant = fp_variant;
   g->g.bp_variant = bp_variant;
   return SK_OK;
@@ -325,15 +325,18 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
     SK_HIP(c, hipEventElapsedTime(ms, e0, e1));
     return SK_OK;
   };
+  const bool verbose = getenv("SYNKRO_VERBOSE") != nullptr;
   float best = 1e30f, ms = 0.0f;
-  for (int v = 0; v <= 1 && !rc; v++) {
+  for (int v = 0; v <= 2 && !rc; v++) {
     rc = time_it(true, v, &ms);
     if (!rc && ms < best) { best = ms; best_fp = v; }
+    if (!rc && verbose) fprintf(stderr, "sk_autotune: fp variant %d: %.3f ms / 3\n", v, ms);
   }
   best = 1e30f;
   for (int v = 0; v <= 2 && !rc; v++) {
     rc = time_it(false, v, &ms);
     if (!rc && ms < best) { best = ms; best_bp = v; }
+    if (!rc && verbose) fprintf(stderr, "sk_autotune: bp variant %d: %.3f ms / 3\n", v, ms);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

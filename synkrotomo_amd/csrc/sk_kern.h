@@ -141,6 +141,55 @@ struct FpParams {
   const float* rowinv;  // FP_SIRT: [A][R]
 };
 
+// Fast-mode row walk of forwardprojection.fut:37-56 for RPT rays of one line, with the
+// line's slope sign as a template argument (Xmax = max(floor xbot, floor xtop) is then
+// floor xtop for slope >= 0, floor xbot otherwise).  Per row the ray contributes
+// fb*v(Xb) + (1 - fb)*v(Xt), fb = (Xmax - xbot)/(xtop - xbot) the fraction of the row's
+// length in the bottom pixel, accumulated as v(Xt) + fb*(v(Xb) - v(Xt)): a ray inside one
+// pixel (Xb == Xt, the reference's full-length case) adds that pixel once whatever fb is.
+// fb is clamped to [0, 1], which also maps the 0/0 of a zero-length range (slope 0) to 0.
+// The sums are fractions of the row length; the caller scales by lbase.
+template <bool POS, int RPT>
+__device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int i0, int i1,
+                                             float slope, const float* icpt, float* sum) {
+  if (i0 >= i1) return;
+  // Software-pipelined, unrolled by two rows so no register copies wait on a load: row i+1's
+  // gathers are issued before row i's fractions are formed, so a wave keeps 2*RPT loads in
+  // flight across its own arithmetic.  A prefetch past the last row uses an empty descriptor
+  // (no memory access, loads return 0).
+  float xb0[RPT], xt0[RPT], vb0[RPT], vt0[RPT];
+  float xb1[RPT], xt1[RPT], vb1[RPT], vt1[RPT];
+  auto gather = [&](int i, float* xbo, float* xto, float* vbo, float* vto) {
+    const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, i < i1 ? N : 0);
+    const float yslope = (float)i * slope;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      xbo[k] = yslope + icpt[k];
+      xto[k] = xbo[k] + slope;
+      vbo[k] = row_load(row, (int)floorf(xbo[k]) + h);
+      vto[k] = row_load(row, (int)floorf(xto[k]) + h);
+    }
+  };
+  auto consume = [&](const float* xbi, const float* xti, const float* vbi, const float* vti) {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const float Xmax = floorf(POS ? xti[k] : xbi[k]);
+      const float fb = fminf(fmaxf((Xmax - xbi[k]) * fast_rcp(xti[k] - xbi[k]), 0.0f), 1.0f);
+      sum[k] = sum[k] + vti[k];
+      sum[k] = fmaf(fb, vbi[k] - vti[k], sum[k]);
+    }
+  };
+  gather(i0, xb0, xt0, vb0, vt0);
+  int i = i0;
+  for (; i + 1 < i1; i += 2) {
+    gather(i + 1, xb1, xt1, vb1, vt1);
+    consume(xb0, xt0, vb0, vt0);
+    gather(i + 2, xb0, xt0, vb0, vt0);
+    consume(xb1, xt1, vb1, vt1);
+  }
+  if (i < i1) consume(xb0, xt0, vb0, vt0);
+}
+
 // One wave's work item: line j, bins [wb*64*RPT, (wb+1)*64*RPT) of image bz (lane l: bins
 // first + l + 64k), all lanes stepping the same frame row.  WEIGHTS == W_ADJ walks the same
 // rays with the backprojector's lengths (B^T).  No barriers: waves are independent.
@@ -172,55 +221,54 @@ __device__ __forceinline__ void fp_item(const FpParams& P, int j, int wb, int bz
   }
   const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
 
-  for (int i = i0; i < i1; ++i) {
-    const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, N);
-    const float yslope = (float)i * slope;
+  constexpr bool FASTW = !(WEIGHTS == W_ADJ || EXACT || MODE == FP_ONES_INV);
+  if constexpr (FASTW) {
+    if (L.flags & LINE_SLOPE_POS)
+      fp_rows_fast<true, RPT>(src, N, h, i0, i1, slope, icpt, sum);
+    else
+      fp_rows_fast<false, RPT>(src, N, h, i0, i1, slope, icpt, sum);
+  } else {
+    for (int i = i0; i < i1; ++i) {
+      const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, N);
+      const float yslope = (float)i * slope;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      // forwardprojection.fut:37-56
-      const float xbot = yslope + icpt[k];
-      const float xtop = xbot + slope;
-      const float fXb = floorf(xbot);
-      const float fXt = floorf(xtop);
-      const int Xb = (int)fXb;
-      const int Xt = (int)fXt;
-      const bool single = Xb == Xt;
-      float vb = 1.0f, vt = 1.0f;
-      if constexpr (MODE != FP_ONES_INV) {
-        vb = row_load(row, Xb + h);
-        vt = row_load(row, Xt + h);
-      }
-      if constexpr (WEIGHTS == W_ADJ || EXACT || MODE == FP_ONES_INV) {
-        float lb, lt;
-        if constexpr (WEIGHTS == W_ADJ) {
-          lb = bp_entry<EXACT, DR1>(L, Xb, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
-          lt = bp_entry<EXACT, DR1>(L, Xt, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
-        } else {
-          const float Xmax = fmaxf(fXb, fXt);
-          const float xdiff = xtop - xbot;
-          if constexpr (EXACT) {
-            lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
-            lt = ((xtop - Xmax) / xdiff) * lbase;
-          } else {
-            const float rd = fast_rcp(xdiff);
-            lb = single ? lbase : ((Xmax - xbot) * rd) * lbase;
-            lt = ((xtop - Xmax) * rd) * lbase;
-          }
+      for (int k = 0; k < RPT; ++k) {
+        // forwardprojection.fut:37-56
+        const float xbot = yslope + icpt[k];
+        const float xtop = xbot + slope;
+        const float fXb = floorf(xbot);
+        const float fXt = floorf(xtop);
+        const int Xb = (int)fXb;
+        const int Xt = (int)fXt;
+        const bool single = Xb == Xt;
+        float vb = 1.0f, vt = 1.0f;
+        if constexpr (MODE != FP_ONES_INV) {
+          vb = row_load(row, Xb + h);
+          vt = row_load(row, Xt + h);
         }
-        const bool inb = (unsigned)(Xb + h) < (unsigned)N;
-        const bool intop = !single && (unsigned)(Xt + h) < (unsigned)N;
-        const float bot = inb ? lb * vb : 0.0f;
-        const float top = intop ? lt * vt : 0.0f;
-        sum[k] = sum[k] + (bot + top);
-      } else {
-        // Fast mode: the fractions of the row's length (1 in a single pixel), scaled by
-        // lbase once per ray.  Out-of-bounds columns load 0.
-        const float Xmax = fmaxf(fXb, fXt);
-        const float rd = fast_rcp(xtop - xbot);
-        const float fb = single ? 1.0f : (Xmax - xbot) * rd;
-        const float ft = single ? 0.0f : (xtop - Xmax) * rd;
-        sum[k] = fmaf(fb, vb, sum[k]);
-        sum[k] = fmaf(ft, vt, sum[k]);
+        {
+          float lb, lt;
+          if constexpr (WEIGHTS == W_ADJ) {
+            lb = bp_entry<EXACT, DR1>(L, Xb, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
+            lt = bp_entry<EXACT, DR1>(L, Xt, (float)i, xbot, xtop, r[k], P.ncand, P.rho0, P.drho);
+          } else {
+            const float Xmax = fmaxf(fXb, fXt);
+            const float xdiff = xtop - xbot;
+            if constexpr (EXACT) {
+              lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
+              lt = ((xtop - Xmax) / xdiff) * lbase;
+            } else {
+              const float rd = fast_rcp(xdiff);
+              lb = single ? lbase : ((Xmax - xbot) * rd) * lbase;
+              lt = ((xtop - Xmax) * rd) * lbase;
+            }
+          }
+          const bool inb = (unsigned)(Xb + h) < (unsigned)N;
+          const bool intop = !single && (unsigned)(Xt + h) < (unsigned)N;
+          const float bot = inb ? lb * vb : 0.0f;
+          const float top = intop ? lt * vt : 0.0f;
+          sum[k] = sum[k] + (bot + top);
+        }
       }
     }
   }
@@ -245,6 +293,7 @@ __device__ __forceinline__ void fp_item(const FpParams& P, int j, int wb, int bz
 struct BpParams {
   const Line* lines;
   int A, R, N, ncand, W;
+  int nsteep;           // lines [0, nsteep) are steep, the rest flat
   float rho0, drho;
   const float* sino;    // [batch][A][R]
   float* out;           // [batch][N][N]  (BP_SIRT: the image, updated in place)
@@ -260,16 +309,34 @@ __host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
   return (W + 1) & ~1;
 }
 
+// Per-line values of one line block, in LDS.  w0: the tile's rho-window origin.  Fast mode
+// only: ci, si, cst give the candidate index t = ci*(xmin+0.5) + si*(ymin+0.5) + cst (the
+// reference's ((cost*x + sint*y) - rho0)/drho, x = xmin+0.5-fact*cost, y = ymin+0.5-fact*sint,
+// with the constant terms folded), [lo, hi] the window's candidate range as floats.  t does
+// not depend on the tile, so every tile shape selects the same candidates.
+struct alignas(16) BpLineLds {
+  int w0;
+  float ci, si, cst;
+  float lo, hi;
+  int pad0, pad1;
+};
+
 // One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
 // (ty, tx) owns pixels (Y0 + ty + 16k, X0 + tx).  Lines are walked in order in blocks of
 // BP_LB: the block finds each line's rho-window origin from the tile's corners (the
 // candidate index is monotone in each frame coordinate), stages the window -- (s*drho +
 // rho0)/cost and the sinogram value -- in LDS, then every lane evaluates its candidates
 // from LDS.  Steep and flat lines accumulate separately and are added at the end
-// (backprojection.fut:75-78).  win: BP_LB*W float2, w0s: BP_LB ints, both in LDS.
+// (backprojection.fut:75-78).  win: BP_LB*W float2, lsh: BP_LB entries, both in LDS.
+//
+// Exact mode evaluates every expression of backprojection.fut:49-62 in the reference's
+// order.  Fast mode keeps the reference's xbot/xtop (so each length fraction is the
+// reference's up to the reciprocal) but computes the candidate index with fewer roundings:
+// a pixel whose rounded index moves by one bin gains or loses only a bin at distance
+// >= sqrt(2)/2 from its centre, where the intersection length is zero.
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
 __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_y, int bz,
-                                        float2* win, int* w0s) {
+                                        float2* win, BpLineLds* lsh) {
   // Fast reference weights: slope sign hoisted per line, lbase folded into the staged value.
   constexpr bool FASTREF = !EXACT && WEIGHTS == W_REF;
   const int ncand = NC > 0 ? NC : P.ncand;
@@ -301,86 +368,144 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
       smin = fminf(smin, cand_s<DR1>(L, cx1, cy0, P.rho0, P.drho));
       smin = fminf(smin, cand_s<DR1>(L, cx0, cy1, P.rho0, P.drho));
       smin = fminf(smin, cand_s<DR1>(L, cx1, cy1, P.rho0, P.drho));
-      w0s[threadIdx.x] = (int)smin;
+      BpLineLds e;
+      e.w0 = (int)smin;
+      e.ci = e.si = e.cst = 0.0f;
+      e.lo = smin;
+      e.hi = smin + (float)(W - ncand);
+      e.pad0 = e.pad1 = 0;
+      if constexpr (FASTREF) {
+        const float idr = DR1 ? 1.0f : 1.0f / P.drho;
+        e.ci = L.cost * idr;
+        e.si = L.sint * idr;
+        e.cst = -(fmaf(L.cost, L.fc, L.sint * L.fs) + P.rho0) * idr;
+      }
+      lsh[threadIdx.x] = e;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nl * W; e += 256) {
-      const int jj = e / W, k = e - jj * W;
+    // One wave per line (line values in scalar registers), lanes over the window's bins.
+    for (int jj = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); jj < nl; jj += 4) {
       const Line L = P.lines[l0 + jj];
-      const int sp = w0s[jj] + k;
-      // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost
-      const float inter = ((float)sp * P.drho + P.rho0) / L.cost;
-      float p = 0.0f;
-      if (sp >= 0 && sp < P.R) p = (MODE == BP_ONES_INV) ? 1.0f : sb[(size_t)L.orig * P.R + sp];
-      if constexpr (FASTREF) p = p * L.lbase;
-      win[jj * W + k] = make_float2(inter, p);
+      const int w0 = lsh[jj].w0;
+      const float* srow = sb + (size_t)L.orig * P.R;
+      for (int k = threadIdx.x & 63; k < W; k += 64) {
+        const int sp = w0 + k;
+        // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost
+        const float inter = ((float)sp * P.drho + P.rho0) / L.cost;
+        float p = 0.0f;
+        if (sp >= 0 && sp < P.R) p = (MODE == BP_ONES_INV) ? 1.0f : srow[sp];
+        if constexpr (FASTREF) p = p * L.lbase;
+        win[jj * W + k] = make_float2(inter, p);
+      }
     }
     __syncthreads();
-    for (int jj = 0; jj < nl; ++jj) {
-      const Line L = P.lines[l0 + jj];
-      const bool fl = L.flags & LINE_FLAT;
-      const bool degen = L.flags & LINE_DEGEN;
-      const bool pos = L.flags & LINE_SLOPE_POS;
-      const int w0 = w0s[jj];
+    // Steep lines precede flat lines in the table, so the block's lines split into (at
+    // most) one steep and one flat run; each run uses its own frame coordinates.
+    const int nsteep = min(max(P.nsteep - l0, 0), nl);
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      const int j_begin = part == 0 ? 0 : nsteep, j_end = part == 0 ? nsteep : nl;
+      if (j_begin >= j_end) continue;
+      float xm[PPT], ym[PPT], xh[PPT], yh[PPT], x1[PPT], acc[PPT];
 #pragma unroll
       for (int q = 0; q < PPT; ++q) {
-        const float xmin = fl ? ys[q] : xs[q];
-        const float ymin = fl ? xs[q] : ys[q];
-        const float s = cand_s<DR1>(L, xmin, ymin, P.rho0, P.drho);
-        const float xbase = ymin * L.slope;  // backprojection.fut:55
-        // s - w0 lies in [0, W - ncand] (monotone s, window sized on the host); the clamp
-        // only guards geometries so far off-centre that rho rounds by whole bins.
-        const int off = min(max((int)s - w0, 0), W - ncand);
-        const float2* wrow = win + jj * W + off;
-        const float xmin1 = xmin + 1.0f;
-        float inner = 0.0f;
-        if (FASTREF && !degen) {
-          if (pos) {
+        xm[q] = part ? ys[q] : xs[q];
+        ym[q] = part ? xs[q] : ys[q];
+        xh[q] = xm[q] + 0.5f;
+        yh[q] = ym[q] + 0.5f;
+        x1[q] = xm[q] + 1.0f;
+        acc[q] = part ? acc_f[q] : acc_s[q];
+      }
+      for (int jj = j_begin; jj < j_end; ++jj) {
+        const Line L = P.lines[l0 + jj];
+        const BpLineLds lc = lsh[jj];
+        const bool degen = L.flags & LINE_DEGEN;
+        const bool pos = L.flags & LINE_SLOPE_POS;
+        // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
+        const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
+        // window entry of absolute bin s: wline[s] (w0 is uniform: scalar base arithmetic)
+        const float2* wline = win + __builtin_amdgcn_readfirstlane(jj * W - lc.w0);
 #pragma unroll
-            for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
+        for (int q = 0; q < PPT; ++q) {
+          const float xmin = xm[q], ymin = ym[q], xmin1 = x1[q];
+          const float xbase = ymin * L.slope;  // backprojection.fut:55
+          const float2* wrow;
+          if constexpr (FASTREF) {
+            const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
+            wrow = wline + (int)ceilf(__builtin_amdgcn_fmed3f(t, lc.lo, lc.hi));
+          } else {
+            // backprojection.fut:51-54 with (xmin + 0.5), (ymin + 0.5) hoisted
+            const float px = xh[q] - L.fc;
+            const float py = yh[q] - L.fs;
+            const float rho = L.cost * px + L.sint * py;
+            float t = rho - P.rho0;
+            if constexpr (!DR1) t = t / P.drho;
+            // s - w0 lies in [0, W - ncand] (monotone s, window sized on the host); the
+            // clamp only guards geometries so far off-centre that rho rounds by whole bins.
+            wrow = win + jj * W + min(max((int)ceilf(t) - lc.w0, 0), W - ncand);
+          }
+          constexpr int KC = NC > 0 ? NC : 16;
+          float2 ev[KC];  // fast mode: all candidates read before the per-line branches
+          if constexpr (FASTREF) {
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+              if (NC > 0 || k < ncand) ev[k] = wrow[k];
+          }
+          if (FASTREF && !degen) {
+            // Non-degenerate lines: max(min(a, b), 0) == med3(a, b, 0) because a and b
+            // cannot both be negative when the ray's x-range has positive length; the
+            // fraction m/diff is then >= 0, so min(., 1) is a clamp to [0, 1].
+            if (pos) {
+#pragma unroll
+              for (int k = 0; k < KC; ++k) {
+                if (NC == 0 && k >= ncand) break;
+                const float xb = xbase + ev[k].x;
+                const float xt = xb + L.slope;
+                const float m = __builtin_amdgcn_fmed3f(xt - xmin, xmin1 - xb, 0.0f);
+                const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
+                acc[q] = fmaf(f, ev[k].y, acc[q]);
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < KC; ++k) {
+                if (NC == 0 && k >= ncand) break;
+                const float xb = xbase + ev[k].x;
+                const float xt = xb + L.slope;
+                const float m = __builtin_amdgcn_fmed3f(xb - xmin, xmin1 - xt, 0.0f);
+                const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
+                acc[q] = fmaf(f, ev[k].y, acc[q]);
+              }
+            }
+          } else if (FASTREF) {
+            // degenerate line: general form with the m == 0 case, lbase already in e.y
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
               if (NC == 0 && k >= ncand) break;
-              const float2 e = wrow[k];
-              const float xb = xbase + e.x;
-              const float xt = xb + L.slope;
-              const float m = fmaxf(fminf(xt - xmin, xmin1 - xb), 0.0f);
-              const float f = fminf(m * fast_rcp(xt - xb), 1.0f);
-              inner = fmaf(f, e.y, inner);
+              const float l =
+                  cand_len<false, W_REF>(xbase + ev[k].x, L.slope, 1.0f, xmin, xmin1, true);
+              acc[q] = fmaf(l, ev[k].y, acc[q]);
             }
           } else {
+            // reference order: the line's candidates summed, then added to the pixel
+            float inner = 0.0f;
 #pragma unroll
             for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
               if (NC == 0 && k >= ncand) break;
               const float2 e = wrow[k];
-              const float xb = xbase + e.x;
-              const float xt = xb + L.slope;
-              const float m = fmaxf(fminf(xb - xmin, xmin1 - xt), 0.0f);
-              const float f = fminf(m * fast_rcp(xb - xt), 1.0f);
-              inner = fmaf(f, e.y, inner);
+              const float l =
+                  cand_len<EXACT, WEIGHTS>(xbase + e.x, L.slope, L.lbase, xmin, xmin1, degen);
+              inner = inner + l * e.y;
             }
-          }
-        } else if (FASTREF) {
-          // degenerate line: general form with the m == 0 case, lbase already in e.y
-#pragma unroll
-          for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
-            if (NC == 0 && k >= ncand) break;
-            const float2 e = wrow[k];
-            const float l = cand_len<false, W_REF>(xbase + e.x, L.slope, 1.0f, xmin, xmin1, true);
-            inner = fmaf(l, e.y, inner);
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < (NC > 0 ? NC : 16); ++k) {
-            if (NC == 0 && k >= ncand) break;
-            const float2 e = wrow[k];
-            const float l =
-                cand_len<EXACT, WEIGHTS>(xbase + e.x, L.slope, L.lbase, xmin, xmin1, degen);
-            inner = inner + l * e.y;
+            acc[q] = acc[q] + inner;
           }
         }
-        if (fl)
-          acc_f[q] = acc_f[q] + inner;
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        if (part)
+          acc_f[q] = acc[q];
         else
-          acc_s[q] = acc_s[q] + inner;
+          acc_s[q] = acc[q];
       }
     }
     __syncthreads();

@@ -54,8 +54,8 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256) void k_bp(BpParams P) {
   extern __shared__ float2 win[];
-  __shared__ int w0s[BP_LB];
-  bp_item<EXACT, MODE, WEIGHTS, NC, DR1, PPT>(P, blockIdx.x, blockIdx.y, blockIdx.z, win, w0s);
+  __shared__ BpLineLds lsh[BP_LB];
+  bp_item<EXACT, MODE, WEIGHTS, NC, DR1, PPT>(P, blockIdx.x, blockIdx.y, blockIdx.z, win, lsh);
 }
 
 // ------------------------------------------------------------------------ launchers
@@ -76,7 +76,7 @@ FpParams fp_params(const Geom* g, const FpArgs& a) {
                   a.img, a.imgT, a.out, a.b, a.rowinv};
 }
 
-int fp_rays_per_lane(int variant) { return variant == 1 ? 2 : 1; }
+int fp_rays_per_lane(int variant) { return variant == 1 ? 2 : (variant == 2 ? 4 : 1); }
 int bp_pixels_per_lane(int variant) { return variant == 1 ? 2 : (variant == 2 ? 4 : 1); }
 
 template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
@@ -91,6 +91,8 @@ template <bool EXACT, int MODE>
 static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) {
   if (!EXACT && variant == 1)
     fp_go<EXACT, MODE, W_REF, true, 2>(g, a, s);
+  else if (!EXACT && variant == 2)
+    fp_go<EXACT, MODE, W_REF, true, 4>(g, a, s);
   else
     fp_go<EXACT, MODE, W_REF, true, 1>(g, a, s);
 }
@@ -120,7 +122,7 @@ int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s) {
 
 BpParams bp_params(const Geom* g, const BpArgs& a, int ppt) {
   return BpParams{g->d_lines, g->A, g->R, g->N, g->ncand, bp_window_bins(ppt, g->drho, g->ncand),
-                  g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv};
+                  g->n_steep, g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv};
 }
 
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>

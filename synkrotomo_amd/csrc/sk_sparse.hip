@@ -12,17 +12,28 @@
 //                  [0, R) with non-zero intersect_fact length (backprojection.fut:44-78).
 //                  B is the reference's backprojector, which is A^T except for edge rays on
 //                  lines whose slope rounds to zero.
+//   A^T (SK_MAT_ADJ): the CSC form of A, stored as the CSR of A^T: rows = pixels, columns =
+//                  rays.  Entries per pixel: steep lines then flat lines, rays in bin order,
+//                  exactly A's entries of that pixel (explicit zeros included) with A's
+//                  values bit for bit: the same xbot = y*slope + (s*drho + rho0)/cost and
+//                  forwardprojection.fut:37-54 lengths, evaluated for the backprojector's
+//                  candidate window widened by one bin on each side (a ray with an entry in
+//                  the pixel touches the pixel square, so it lies within sqrt(2)/2 of the
+//                  pixel centre; the margin absorbs the rounding of the window origin).
 // CSR, 64-bit row pointers (nnz exceeds 2^31 at 2048^2), 32-bit columns, f32 values.
 //
 // Build = count pass -> 64-bit exclusive scan -> fill pass.  Both passes are
 // wavefront-cooperative: one wave per row, lanes over the row's steps (A: 64 image rows at
 // a time; B: 64 lines at a time); ballots and mbcnt place each lane's entries, so the
 // entries come out in the sequential order above and the writes are contiguous.
+#include <cstdlib>
+
 #include "sk_common.h"
 
 struct sk_mat {
   int kind = 0;
-  int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep); -1: not yet timed
+  int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
+                          // 16-byte streams; -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
@@ -93,8 +104,8 @@ __global__ __launch_bounds__(SP_BLOCK) void k_fp_rows(const Line* __restrict__ l
   if (!FILL && lane == 0) counts[row] = total;
 }
 
-// ------------------------------------------------------------------ B: one wave per pixel
-template <bool FILL>
+// ------------------------------------------------------------------ B, A^T: one wave per pixel
+template <bool FILL, bool ADJ>
 __global__ __launch_bounds__(SP_BLOCK) void k_bp_rows(const Line* __restrict__ lines, int A, int R,
                                                       int N, float rho0, float drho, int ncand,
                                                       int32_t* __restrict__ counts,
@@ -123,24 +134,41 @@ __global__ __launch_bounds__(SP_BLOCK) void k_bp_rows(const Line* __restrict__ l
     const float xbase = ymin * L.slope;
     int before = 0;  // entries of lower lanes
     int mine = 0;    // entries of this lane so far
-    unsigned long long masks[16];
-    float lens[16];
-    int sps[16];
-    for (int k = 0; k < ncand; ++k) {
+    constexpr int KMAX = 16 + (ADJ ? 2 : 0);
+    const int nk = ncand + (ADJ ? 2 : 0);
+    unsigned long long masks[KMAX];
+    float lens[KMAX];
+    int sps[KMAX];
+    for (int k = 0; k < nk; ++k) {
       // backprojection.fut:56-64
-      const float sp = s + (float)k;
+      const float sp = s + (float)(ADJ ? k - 1 : k);
       const float rr = sp * drho + rho0;
       const float xbot = xbase + (rr / L.cost);
       const float xtop = xbot + L.slope;
-      const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
-      const float lo = fmaxf(maxx - xmin, 0.0f);
-      const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
-      const float diff = maxx - minx;
-      const float m = fminf(lo, hi);
-      const float p = (m == 0.0f) ? 0.0f : m / diff;
-      const float l = fminf(p, 1.0f) * L.lbase;
+      float l;
+      bool member;
+      if constexpr (ADJ) {
+        // the entry of A for (ray sp, this pixel): forwardprojection.fut:37-54
+        const float fXb = floorf(xbot), fXt = floorf(xtop);
+        const bool single = fXb == fXt;
+        const float Xmax = fmaxf(fXb, fXt);
+        const float xdiff = xtop - xbot;
+        const float lb = single ? L.lbase : ((Xmax - xbot) / xdiff) * L.lbase;
+        const float lt = ((xtop - Xmax) / xdiff) * L.lbase;
+        member = fXb == xmin || (!single && fXt == xmin);
+        l = fXb == xmin ? lb : lt;
+      } else {
+        const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
+        const float lo = fmaxf(maxx - xmin, 0.0f);
+        const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
+        const float diff = maxx - minx;
+        const float m = fminf(lo, hi);
+        const float p = (m == 0.0f) ? 0.0f : m / diff;
+        l = fminf(p, 1.0f) * L.lbase;
+        member = l != 0.0f;
+      }
       const int si = (int)sp;
-      const bool ok = live && si >= 0 && si < R && l != 0.0f;
+      const bool ok = live && si >= 0 && si < R && member;
       masks[k] = __ballot(ok);
       lens[k] = l;
       sps[k] = ok ? si : -1;
@@ -148,7 +176,7 @@ __global__ __launch_bounds__(SP_BLOCK) void k_bp_rows(const Line* __restrict__ l
     }
     if (FILL) {
       long p = pos + before;
-      for (int k = 0; k < ncand; ++k) {
+      for (int k = 0; k < nk; ++k) {
         if (sps[k] >= 0) {
           cols[p + mine] = L.orig * R + sps[k];
           vals[p + mine] = lens[k];
@@ -157,7 +185,7 @@ __global__ __launch_bounds__(SP_BLOCK) void k_bp_rows(const Line* __restrict__ l
       }
     }
     int n = 0;
-    for (int k = 0; k < ncand; ++k) n += __popcll(masks[k]);
+    for (int k = 0; k < nk; ++k) n += __popcll(masks[k]);
     pos += n;
     total += n;
   }
@@ -260,6 +288,52 @@ __global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr
   if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
 }
 
+// ------------------------------------------------------------------ SpMV: one wave per row, wide
+// As k_spmv, but each lane streams aligned 4-entry chunks (16-byte column and value loads)
+// and keeps U chunks -- 4U gathers -- in flight per iteration; the at most 3 entries before
+// the first aligned chunk and after the last are handled by single lanes.
+template <int U>
+__global__ __launch_bounds__(256) void k_spmv_wide(const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col,
+                                                   const float* __restrict__ val, long rows,
+                                                   long ncols, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+  const int lane = lane_id();
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const long beg = rowptr[row], end = rowptr[row + 1];
+  const long a0 = min((beg + 3) & ~3L, end);  // first aligned entry (or end)
+  const long a1 = max(a0, end & ~3L);         // end of the aligned body
+  float acc = 0.0f;
+  if (beg + lane < a0) acc = fmaf(val[beg + lane], xb[col[beg + lane]], acc);
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const i32x4* c4 = reinterpret_cast<const i32x4*>(col);
+  const f32x4* v4 = reinterpret_cast<const f32x4*>(val);
+  const long q1 = a1 >> 2;
+  for (long q = (a0 >> 2) + lane; q < q1; q += 64 * U) {
+    i32x4 c[U];
+    f32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = q + 64 * u < q1;
+      c[u] = ok ? __builtin_nontemporal_load(c4 + q + 64 * u) : i32x4{0, 0, 0, 0};
+      v[u] = ok ? __builtin_nontemporal_load(v4 + q + 64 * u) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      acc = fmaf(v[u].x, xb[c[u].x], acc);
+      acc = fmaf(v[u].y, xb[c[u].y], acc);
+      acc = fmaf(v[u].z, xb[c[u].z], acc);
+      acc = fmaf(v[u].w, xb[c[u].w], acc);
+    }
+  }
+  if (a1 + lane < end) acc = fmaf(val[a1 + lane], xb[col[a1 + lane]], acc);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
 // counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
 static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr, int64_t* nnz,
                        hipStream_t s) {
@@ -329,13 +403,20 @@ static void launch_rows(const Geom& G, int kind, bool fill, int32_t* counts, con
     else
       hipLaunchKernelGGL(k_fp_rows<false>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
                          G.rho0, G.drho, counts, rowptr, cols, vals);
+  } else if (kind == SK_MAT_BP) {
+    if (fill)
+      hipLaunchKernelGGL((k_bp_rows<true, false>), dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R,
+                         G.N, G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
+    else
+      hipLaunchKernelGGL((k_bp_rows<false, false>), dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R,
+                         G.N, G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
   } else {
     if (fill)
-      hipLaunchKernelGGL(k_bp_rows<true>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
-                         G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
+      hipLaunchKernelGGL((k_bp_rows<true, true>), dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R,
+                         G.N, G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
     else
-      hipLaunchKernelGGL(k_bp_rows<false>, dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R, G.N,
-                         G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
+      hipLaunchKernelGGL((k_bp_rows<false, true>), dim3(blocks), dim3(SP_BLOCK), 0, s, G.d_lines, G.A, G.R,
+                         G.N, G.rho0, G.drho, G.ncand, counts, rowptr, cols, vals);
   }
 }
 
@@ -358,7 +439,8 @@ int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stre
   Ctx* c = &ctx->c;
   *out = nullptr;
   if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
-  if (which != SK_MAT_FP && which != SK_MAT_BP) return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
+  if (which != SK_MAT_FP && which != SK_MAT_BP && which != SK_MAT_ADJ)
+    return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
   const Geom& G = g->g;
   if ((long)G.A * G.R >= (1L << 31) || (long)G.N * G.N >= (1L << 31))
     return fail(c, SK_EINVAL, "matrix dimensions exceed 32-bit column indices");
@@ -412,7 +494,8 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
   if (!ctx || !nnz_out) return SK_EINVAL;
   Ctx* c = &ctx->c;
   if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
-  if (which != SK_MAT_FP && which != SK_MAT_BP) return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
+  if (which != SK_MAT_FP && which != SK_MAT_BP && which != SK_MAT_ADJ)
+    return fail(c, SK_EINVAL, "unknown matrix kind %d", which);
   const Geom& G = g->g;
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
@@ -434,7 +517,7 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 1) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 2) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -483,7 +566,11 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   auto run = [&](int v) {
-    if (v == 1) {
+    if (v == 2) {
+      dim3 grid((unsigned)((m->rows + 3) / 4), batch);
+      hipLaunchKernelGGL(k_spmv_wide<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                         (long)m->rows, (long)m->cols, d_in, d_out);
+    } else if (v == 1) {
       dim3 grid((unsigned)((m->rows + 255) / 256), batch);
       hipLaunchKernelGGL(k_spmv_lockstep, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
                          (long)m->rows, (long)m->cols, d_in, d_out);
@@ -494,20 +581,32 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     }
   };
   if (m->spmv_variant < 0) {
-    // First use: time both kernels once on this matrix and keep the faster.
-    hipEvent_t e[3];
+    // First use: time the kernels once on this matrix and keep the fastest.
+    constexpr int NV = 3;
+    hipEvent_t e[NV + 1];
     for (auto& ev : e) SK_HIP(c, hipEventCreate(&ev));
     SK_HIP(c, hipEventRecord(e[0], s));
-    run(0);
-    SK_HIP(c, hipEventRecord(e[1], s));
-    run(1);
-    SK_HIP(c, hipEventRecord(e[2], s));
-    SK_HIP(c, hipEventSynchronize(e[2]));
-    float t0 = 0.0f, t1 = 0.0f;
-    SK_HIP(c, hipEventElapsedTime(&t0, e[0], e[1]));
-    SK_HIP(c, hipEventElapsedTime(&t1, e[1], e[2]));
+    for (int v = 0; v < NV; ++v) {
+      run(v);
+      SK_HIP(c, hipEventRecord(e[v + 1], s));
+    }
+    SK_HIP(c, hipEventSynchronize(e[NV]));
+    int best = 0;
+    float tbest = 0.0f;
+    for (int v = 0; v < NV; ++v) {
+      float t = 0.0f;
+      SK_HIP(c, hipEventElapsedTime(&t, e[v], e[v + 1]));
+      if (getenv("SYNKRO_VERBOSE")) fprintf(stderr, "sk_spmv: variant %d: %.3f ms\n", v, t);
+      if (v == 0 || t < tbest) {
+        tbest = t;
+        best = v;
+      }
+    }
     for (auto& ev : e) (void)hipEventDestroy(ev);
-    const_cast<sk_mat*>(m)->spmv_variant = t1 < t0 ? 1 : 0;
+    const_cast<sk_mat*>(m)->spmv_variant = best;
+    // The kernels sum a row in different orders; leave the chosen one's result, so that
+    // every call on this matrix returns the same bits.
+    if (best != NV - 1) run(best);
     return launched(c, "SpMV kernel");
   }
   run(m->spmv_variant);

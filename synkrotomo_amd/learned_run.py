@@ -213,7 +213,7 @@ def main(argv=None) -> int:
     t.add_argument("--phantoms", type=int, default=20000)
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=2e-3)
-    t.add_argument("--epochs", type=int, default=1)
+    t.add_argument("--epochs", type=int, default=3)
     t.add_argument("--res-weight", type=float, default=0.0,
                    help="weight of the final data residual ||Ax-b||^2/||b||^2 in the loss")
     t.add_argument("--stage-weight", type=float, default=0.0,

@@ -125,7 +125,8 @@ class Geometry:
             pass
 
     def sparse(self, kind: str) -> "Matrix":
-        """The explicit matrix of this geometry ("fp": A, "bp": B), built once and cached."""
+        """The explicit matrix of this geometry ("fp": A, "bp": B, "adj": A^T), built once and
+        cached."""
         mats = self.__dict__.setdefault("_mats", {})
         m = mats.get(kind)
         if m is None:
@@ -139,7 +140,7 @@ class Geometry:
 
     def sparse_nnz(self, kind: str) -> int:
         """nnz of the explicit matrix without building it."""
-        which = {"fp": nat.SK_MAT_FP, "bp": nat.SK_MAT_BP}[kind]
+        which = _MAT_KINDS[kind]
         v = ctypes.c_int64()
         nat.check(nat.lib().sk_sparse_count(self.ctx.handle, self.handle, which, ctypes.byref(v),
                                             _stream(self.ctx.device)), self.ctx.handle, "sparse count")
@@ -266,15 +267,19 @@ class Geometry:
         return img
 
 
+_MAT_KINDS = {"fp": nat.SK_MAT_FP, "bp": nat.SK_MAT_BP, "adj": nat.SK_MAT_ADJ}
+
+
 class Matrix:
     """An explicit system matrix in CSR form, built on the GPU (``sk_sparse_build``).
 
     kind "fp": A, rows = rays (angle*R + bin), columns = pixels.
     kind "bp": B, rows = pixels, columns = sinogram entries (the reference backprojector).
+    kind "adj": A^T (A in CSC form), rows = pixels, columns = rays; A's entries bit for bit.
     """
 
     def __init__(self, geom: "Geometry", kind: str):
-        which = {"fp": nat.SK_MAT_FP, "bp": nat.SK_MAT_BP}[kind]
+        which = _MAT_KINDS[kind]
         self.geom = geom
         self.kind = kind
         h = ctypes.c_void_p()

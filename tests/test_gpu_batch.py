@@ -14,6 +14,7 @@ def _uniform(A):
 
 
 @pytest.mark.parametrize("N,A,R,rz,dr", [(64, 90, 91, -45.0, 1.0), (512, 360, 725, -362.0, 1.0),
+                                         (1024, 720, 1449, -724.0, 1.0),
                                          (40, 23, 30, -29.0, 2.0), (24, 19, 70, -17.25, 0.5)])
 def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
     geo = ctx.geometry(_uniform(A), rz, dr, R, N)
@@ -21,7 +22,7 @@ def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
     y = torch.rand(2, A, R, device="cuda")
     geo.set_variants(0, 0)
     f0, b0 = geo.fp(x), geo.bp(y)
-    for fv in (0, 1):
+    for fv in (0, 1, 2):
         for bv in (0, 1, 2):
             geo.set_variants(fv, bv)
             torch.testing.assert_close(geo.fp(x), f0, rtol=0, atol=0)
@@ -37,20 +38,26 @@ def test_autotune_cache(ctx, tmp_path, monkeypatch):
     monkeypatch.setenv("SYNKRO_TUNE_CACHE", str(tmp_path / "tune.json"))
     geo = ctx.geometry(_uniform(180), -181.0, 1.0, 363, 256)
     fv, bv = autotune.tuned(geo)
-    assert fv in (0, 1) and bv in (0, 1, 2)
+    assert fv in (0, 1, 2) and bv in (0, 1, 2)
     assert (tmp_path / "tune.json").exists()
     assert autotune.tuned(geo) == (fv, bv)  # served from the cache
     geo.set_variants(-1, -1)
 
 
-def test_sirt_batched_equals_per_problem(ctx):
-    """Uneven groups in one persistent launch sequence == each problem run on its own."""
+@pytest.mark.parametrize("variants", [(-1, -1), (2, 2)])
+@pytest.mark.parametrize("specs", [
+    [(32, 30, 46, -22.5, 3), (64, 45, 91, -45.0, 2), (48, 20, 68, -33.5, 4)],
+    [(256, 180, 363, -181.0, 2), (512, 360, 725, -362.0, 1)],
+], ids=["small", "256-512"])
+def test_sirt_batched_equals_per_problem(ctx, specs, variants):
+    """Uneven groups in one persistent launch sequence == each problem run on its own, with
+    the per-problem runs on the default and on other kernel shapes than the batched ones."""
     from synkrotomo_amd.projector import sirt_batched
-    specs = [(32, 30, 46, -22.5, 3), (64, 45, 91, -45.0, 2), (48, 20, 68, -33.5, 4)]
     gen = torch.Generator(device="cuda").manual_seed(7)
     groups, singles = [], []
     for N, A, R, rz, b in specs:
         geo = ctx.geometry(_uniform(A), rz, 1.0, R, N)
+        geo.set_variants(*variants)
         sino = torch.rand(b, A, R, device="cuda", generator=gen)
         img = torch.zeros(b, N, N, device="cuda")
         groups.append((geo, img, sino))

@@ -50,13 +50,41 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
     x = rng.random(N * N).astype(np.float32)
     y = rng.random(ang.size * R).astype(np.float32)
     geo = ctx.geometry(ang, rz, dr, R, N)
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         for kind in ("fp", "bp"):
             geo.sparse(kind).set_spmv_variant(variant)
         ax = geo.sparse("fp").spmv(_dev(x)).cpu().numpy()
         by = geo.sparse("bp").spmv(_dev(y)).cpu().numpy()
         assert rel_l2(ax, oracle.fp(ang, rz, dr, R, x)) <= TOL
         assert rel_l2(by, oracle.bp(ang, rz, dr, N, y)) <= TOL
+    geo.drop_sparse()
+
+
+def _sorted_entries(rows, cols, vals):
+    order = np.lexsort((cols, rows))
+    return rows[order], cols[order], vals[order]
+
+
+@pytest.mark.parametrize("name,N,ang,R,rz,dr", GEOMS, ids=[g[0] for g in GEOMS])
+def test_adj_is_transpose_of_oracle_A_bitwise(ctx, name, N, ang, R, rz, dr):
+    """SK_MAT_ADJ (A in CSC form) holds exactly the oracle A's entries, explicit zeros
+    included, with bitwise values; its SpMV is the adjoint A^T.y."""
+    rp_ref, c_ref, v_ref = oracle.fp_csr(ang, rz, dr, R, N)
+    ray = np.repeat(np.arange(rp_ref.size - 1), np.diff(rp_ref))
+    want = _sorted_entries(c_ref.astype(np.int64), ray, v_ref)   # (pixel, ray, value)
+    geo = ctx.geometry(ang, rz, dr, R, N)
+    m = geo.sparse("adj")
+    assert (m.rows, m.cols) == (N * N, ang.size * R)
+    rp, c, v = (t.cpu().numpy() for t in m.csr())
+    pix = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    got = _sorted_entries(pix, c.astype(np.int64), v)
+    assert m.nnz == c_ref.size
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+    y = np.random.default_rng(13).random(ang.size * R).astype(np.float32)
+    aty = m.spmv(_dev(y)).cpu().numpy()
+    assert rel_l2(aty, oracle.fp_adjoint(ang, rz, dr, R, N, y)) <= TOL
+    assert geo.sparse_nnz("adj") == m.nnz
     geo.drop_sparse()
 
 
