@@ -153,12 +153,12 @@ template <bool POS, int RPT>
 __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int i0, int i1,
                                              float slope, const float* icpt, float* sum) {
   if (i0 >= i1) return;
-  // Software-pipelined, unrolled by two rows so no register copies wait on a load: row i+1's
-  // gathers are issued before row i's fractions are formed, so a wave keeps 2*RPT loads in
-  // flight across its own arithmetic.  A prefetch past the last row uses an empty descriptor
-  // (no memory access, loads return 0).
+  // Software-pipelined two rows deep, unrolled by three rows so no register copy waits on a
+  // load: rows i+1 and i+2 are in flight while row i's fractions are formed.  A prefetch
+  // past the last row uses an empty descriptor (no memory access, loads return 0).
   float xb0[RPT], xt0[RPT], vb0[RPT], vt0[RPT];
   float xb1[RPT], xt1[RPT], vb1[RPT], vt1[RPT];
+  float xb2[RPT], xt2[RPT], vb2[RPT], vt2[RPT];
   auto gather = [&](int i, float* xbo, float* xto, float* vbo, float* vto) {
     const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, i < i1 ? N : 0);
     const float yslope = (float)i * slope;
@@ -180,14 +180,26 @@ __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int
     }
   };
   gather(i0, xb0, xt0, vb0, vt0);
+  gather(i0 + 1, xb1, xt1, vb1, vt1);
   int i = i0;
-  for (; i + 1 < i1; i += 2) {
-    gather(i + 1, xb1, xt1, vb1, vt1);
+  // sched_barrier keeps the compiler from regrouping the stages (it would otherwise cluster
+  // the loads and wait on all of them at the top of the loop).
+  for (; i + 2 < i1; i += 3) {
+    gather(i + 2, xb2, xt2, vb2, vt2);
+    __builtin_amdgcn_sched_barrier(0);
     consume(xb0, xt0, vb0, vt0);
-    gather(i + 2, xb0, xt0, vb0, vt0);
+    __builtin_amdgcn_sched_barrier(0);
+    gather(i + 3, xb0, xt0, vb0, vt0);
+    __builtin_amdgcn_sched_barrier(0);
     consume(xb1, xt1, vb1, vt1);
+    __builtin_amdgcn_sched_barrier(0);
+    gather(i + 4, xb1, xt1, vb1, vt1);
+    __builtin_amdgcn_sched_barrier(0);
+    consume(xb2, xt2, vb2, vt2);
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (i < i1) consume(xb0, xt0, vb0, vt0);
+  if (i + 1 < i1) consume(xb1, xt1, vb1, vt1);
 }
 
 // One wave's work item: line j, bins [wb*64*RPT, (wb+1)*64*RPT) of image bz (lane l: bins
