@@ -173,6 +173,18 @@ int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, con
  * [-1,1]^2 pixel centres, sum clipped to [0,1] -> d_out [B][N][N] FP32. */
 int sk_phantoms(sk_ctx* ctx, int B, int N, int E, const double* d_params, float* d_out, void* stream);
 
+/* ---------------------------------------------------------------- comparison tools */
+/* sanityCheck.main, diff.main and mse.main (futhark/tools/sanityCheck.fut:1-4, diff.fut:1-7,
+ * mse.fut:1-3) in one pass over two device arrays of n floats, with the reference's tolerance
+ * (0.05) as the parameter `tol`:
+ *   *all_within = every |sane - test| < tol (a NaN difference fails);
+ *   *n_over     = number of indices i > 0 with |sane - test| > tol (diff.fut's filter (> 0)
+ *                 also drops index 0);
+ *   *mse        = sum of the f32 squares (sane - test)**2 / n, summed in FP64.
+ * Any output pointer may be NULL.  Synchronizes `stream`. */
+int sk_compare(sk_ctx* ctx, int64_t n, const float* d_sane, const float* d_test, float tol,
+               int* all_within, int64_t* n_over, float* mse, void* stream);
+
 /* ---------------------------------------------------------------- sparse path */
 /* forwardprojection_sparse.main / backprojection_sparse.main
  * (futhark/forwardprojection_sparse.fut:26-40, futhark/backprojection_sparse.fut:26-37):

@@ -12,6 +12,9 @@ Reference functions restated (file:line in tomograph/synkrotomo):
   sirt             futhark/SIRT.fut:9-49
   sirt3d           futhark/SIRT3D.fut:18-28 (argument defect fixed: rho0 = rhos[0],
                    drho = rhos[1]-rhos[0], numrhos = len(rhos))
+  sanity_check     futhark/tools/sanityCheck.fut:1-4
+  diff             futhark/tools/diff.fut:1-7
+  mse              futhark/tools/mse.fut:1-3
 """
 from __future__ import annotations
 
@@ -61,6 +64,9 @@ def _load():
         "orc_bp_csr": (ctypes.c_int64, [_f32p, c_int, c_float, c_float, c_int, c_int, _i64p, _i32p, _f32p]),
         "orc_fp_adjoint": (None, [_f32p, c_int, c_float, c_float, c_int, c_int, _f32p, _f64p]),
         "orc_bp_oob_nonzero": (ctypes.c_int64, [_f32p, c_int, c_float, c_float, c_int, c_int]),
+        "orc_sanity_check": (c_int, [_f32p, _f32p, ctypes.c_int64, c_float]),
+        "orc_diff": (ctypes.c_int64, [_f32p, _f32p, ctypes.c_int64, c_float]),
+        "orc_mse": (c_float, [_f32p, _f32p, ctypes.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -211,3 +217,28 @@ def fp_adjoint(angles, rhozero, deltarho, R, N, sino):
 def bp_oob_nonzero(angles, rhozero, deltarho, R, N):
     a = _f32(angles)
     return int(_load().orc_bp_oob_nonzero(_p(a, _f32p), a.size, float(rhozero), float(deltarho), int(R), int(N)))
+
+
+def _pair(sane, test):
+    a, b = _f32(sane), _f32(test)
+    if a.size != b.size:
+        raise ValueError("sizes differ")
+    return a, b
+
+
+def sanity_check(sane, test, tol=0.05):
+    """tools/sanityCheck.fut: every |sane - test| < tol."""
+    a, b = _pair(sane, test)
+    return bool(_load().orc_sanity_check(_p(a, _f32p), _p(b, _f32p), a.size, tol))
+
+
+def diff(sane, test, tol=0.05):
+    """tools/diff.fut: count of indices i > 0 with |sane - test| > tol."""
+    a, b = _pair(sane, test)
+    return int(_load().orc_diff(_p(a, _f32p), _p(b, _f32p), a.size, tol))
+
+
+def mse(sane, test):
+    """tools/mse.fut: sequential f32 mean of the squared differences."""
+    a, b = _pair(sane, test)
+    return float(_load().orc_mse(_p(a, _f32p), _p(b, _f32p), a.size))

@@ -348,3 +348,30 @@ int64_t orc_bp_oob_nonzero(const float* angles, int A, float rho0, float drho, i
   free(L);
   return bad;
 }
+
+/*
+ * Comparison tools (futhark/tools/{sanityCheck,diff,mse}.fut), sequential as Futhark C runs them.
+ * sanityCheck.fut:1-4: reduce (&&) true over |s - t| < 0.05.
+ */
+int orc_sanity_check(const float* sane, const float* test, int64_t n, float tol) {
+  int ok = 1;
+  for (int64_t i = 0; i < n; i++) ok = ok && (fabsf(sane[i] - test[i]) < tol);
+  return ok;
+}
+
+/* diff.fut:1-7: length (filter (> 0) (map3 (\s t i -> if |s-t| > 0.05 then i else -1))). */
+int64_t orc_diff(const float* sane, const float* test, int64_t n, float tol) {
+  int64_t count = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t v = fabsf(sane[i] - test[i]) > tol ? i : -1;
+    if (v > 0) count++;
+  }
+  return count;
+}
+
+/* mse.fut:1-3: (reduce (+) 0 (map2 (\s t -> (s-t)**2))) / r32(n), all in f32. */
+float orc_mse(const float* sane, const float* test, int64_t n) {
+  float acc = 0.0f;
+  for (int64_t i = 0; i < n; i++) acc = acc + powf(sane[i] - test[i], 2.0f);
+  return acc / (float)n;
+}

@@ -42,6 +42,10 @@ void orc_fp_adjoint(const float* angles, int A, float rho0, float drho, int R, i
                     const float* sino, double* out);
 int64_t orc_bp_oob_nonzero(const float* angles, int A, float rho0, float drho, int R, int N);
 
+int orc_sanity_check(const float* sane, const float* test, int64_t n, float tol);
+int64_t orc_diff(const float* sane, const float* test, int64_t n, float tol);
+float orc_mse(const float* sane, const float* test, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ SIGNATURES = {
     "sk_geom_destroy": (None, [_vp]),
     "sk_geom_query": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "sk_lines_compute": (_i, [_f32p, _i, _f32p]),
+    "sk_compare": (_i, [_vp, ctypes.c_int64, _vp, _vp, _f, _ip, _i64p, _f32p, _vp]),
     "sk_fp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),

@@ -8,7 +8,7 @@ Modules and classes keep the reference's names and ``main`` argument orders
 (README.md:21-25, Makefile:1-4); each call runs hand-written HIP through libsynkro.so.
 """
 from . import (SIRT, SIRT3D, backprojection, backprojection_sparse,  # noqa: F401
-               forwardprojection, forwardprojection_sparse)
+               forwardprojection, forwardprojection_sparse, tools)
 
 __all__ = ["SIRT", "SIRT3D", "backprojection", "backprojection_sparse", "forwardprojection",
-           "forwardprojection_sparse"]
+           "forwardprojection_sparse", "tools"]

@@ -18,7 +18,9 @@ floor (as consistent with the data as the true image, within 10%).
 
 Time to a target = (iterations or stages needed) x (measured time per iteration or stage
 on the GPU, batch of images); errors are the relative image error against the true
-phantom computed in FP64.
+phantom computed in FP64.  The learned time is reported without and with the fallback
+check (plain SIRT for the same number of steps plus both residuals), which the learned
+method pays on every reconstruction.
 """
 from __future__ import annotations
 
@@ -77,11 +79,20 @@ def train(args) -> int:
         if args.stage_weight > 0:
             for xs in stages[:-1]:
                 loss = loss + args.stage_weight * torch.mean((xs - x_true) ** 2)
-        if args.res_weight > 0:
-            # data consistency of the final image: ||A x - b||^2 / ||b||^2 per image
+        if args.res_weight > 0 or args.res_excess_weight > 0:
+            # data consistency of the final image: ||A x - b|| / ||b|| per image
             from .autograd import project
-            r = (project(x, geo) - b).reshape(b.shape[0], -1)
-            loss = loss + args.res_weight * torch.mean(r.square().sum(1) / b.reshape(b.shape[0], -1).square().sum(1))
+            bb = b.reshape(b.shape[0], -1)
+            bn = bb.square().sum(1)
+            r2 = (project(x, geo).reshape(b.shape[0], -1) - bb).square().sum(1) / bn
+            if args.res_weight > 0:
+                loss = loss + args.res_weight * torch.mean(r2)
+            if args.res_excess_weight > 0:
+                # only the part of the residual above 1.1x the phantom's own noise floor
+                with torch.no_grad():
+                    floor = ((geo.fp(x_true).reshape(b.shape[0], -1) - bb).square().sum(1) / bn).sqrt()
+                excess = torch.relu(r2.clamp_min(1e-20).sqrt() - 1.1 * floor)
+                loss = loss + args.res_excess_weight * torch.mean(excess.square())
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
@@ -96,7 +107,8 @@ def train(args) -> int:
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     net.save(args.out, scale=scale, I0=I0, stages=STAGES, N=N, A=A, phantoms=args.phantoms,
              batch=args.batch, epochs=args.epochs, res_weight=args.res_weight,
-             stage_weight=args.stage_weight, train_seconds=train_s)
+             stage_weight=args.stage_weight, res_excess_weight=args.res_excess_weight,
+             train_seconds=train_s)
     print(json.dumps({"saved": args.out, "train_seconds": round(train_s, 1), "scale": scale}), flush=True)
     return 0
 
@@ -125,16 +137,23 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         for _ in range(3):
             geo.sirt_(xb, b, 1)
             net(model.sirt_step(xb, b))
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        xf = torch.zeros_like(xt)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         e[0].record()
         geo.sirt_(xb, b, 20)
         e[1].record()
         for _ in range(5):
             net(model.sirt_step(xb, b))
         e[2].record()
+        # the fallback check: plain SIRT for the same number of steps and both residuals
+        geo.sirt_(xf, b, STAGES)
+        residual(geo, xb, b)
+        residual(geo, xf, b)
+        e[3].record()
         torch.cuda.synchronize()
     t_iter = e[0].elapsed_time(e[1]) / 20 / batch * 1e-3
     t_stage = e[1].elapsed_time(e[2]) / 5 / batch * 1e-3
+    t_check = e[2].elapsed_time(e[3]) / batch * 1e-3
 
     rows = []
     for start in range(0, n_test, batch):
@@ -178,7 +197,8 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         if verbose:
             print(json.dumps({"done": start + B}), flush=True)
 
-    out = {"t_iter_ms": round(t_iter * 1e3, 4), "t_stage_ms": round(t_stage * 1e3, 4), "scale": scale,
+    out = {"t_iter_ms": round(t_iter * 1e3, 4), "t_stage_ms": round(t_stage * 1e3, 4),
+           "t_fallback_check_ms": round(t_check * 1e3, 4), "scale": scale,
            "n_test": n_test, "max_iters": max_iters,
            "noise_floor_median": float(np.median([r["floor"] for r in rows if r["target"] == "1e-2"]))}
     for tname in ("1e-2", "1.1x_floor"):
@@ -189,10 +209,13 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         if both:
             tp = np.array([r["plain_iters"] * t_iter for r in both])
             tl = np.array([STAGES * t_stage + r["learned_extra_iters"] * t_iter for r in both])
+            tf = tl + t_check
             summary.update({
                 "median_plain_ms": round(float(np.median(tp)) * 1e3, 3),
                 "median_learned_ms": round(float(np.median(tl)) * 1e3, 3),
                 "speedup_median": round(float(np.median(tp) / np.median(tl)), 2),
+                "median_learned_with_check_ms": round(float(np.median(tf)) * 1e3, 3),
+                "speedup_with_check_median": round(float(np.median(tp) / np.median(tf)), 2),
                 "median_plain_iters": float(np.median([r["plain_iters"] for r in both])),
                 "median_learned_extra_iters": float(np.median([r["learned_extra_iters"] for r in both])),
                 "median_plain_err": round(float(np.median([r["plain_err"] for r in both])), 5),
@@ -218,6 +241,9 @@ def main(argv=None) -> int:
                    help="weight of the final data residual ||Ax-b||^2/||b||^2 in the loss")
     t.add_argument("--stage-weight", type=float, default=0.0,
                    help="weight of the image error after each earlier stage")
+    t.add_argument("--res-excess-weight", type=float, default=0.0,
+                   help="weight of the squared excess of the final residual over 1.1x the "
+                        "phantom's noise floor")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     e = sub.add_parser("eval")
     e.add_argument("--phantoms", type=int, default=1000)

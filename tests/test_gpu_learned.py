@@ -47,42 +47,63 @@ def test_forward_matches_cpu_fp32_reference(ctx, B, H, W):
     assert rel_l2(y.cpu().numpy(), ref.numpy()) < 1e-3
 
 
-def _layer_grads(fn, inputs, g):
-    """FP64 autograd of one layer; returns the gradients of every input."""
-    ins = [t.detach().double().clone().requires_grad_(True) for t in inputs]
-    out = fn(*ins)
-    return torch.autograd.grad(out, ins, g.double())
+def _gpu_activations(ctx, net, x):
+    """The kernels' own forward activations h1, h2, h3 (BF16, NHWC) for input x [B, H, W]."""
+    from synkrotomo_amd import _native as nat
+    from synkrotomo_amd.projector import _ptr, _stream
+    L, h, s = nat.lib(), ctx.handle, _stream(ctx.device)
+    B, H, W = x.shape
+    w1b, w2b, w3b = (w.detach().to(torch.bfloat16).contiguous() for w in (net.w1, net.w2, net.w3))
+    h1 = torch.empty((B, H, W, 32), dtype=torch.bfloat16, device="cuda")
+    h2, h3 = torch.empty_like(h1), torch.empty_like(h1)
+    nat.check(L.sk_conv_in(h, B, H, W, _ptr(x), _ptr(w1b), _ptr(net.b1.detach()), _ptr(h1), s), h, "conv1")
+    nat.check(L.sk_conv32(h, B, H, W, _ptr(h1), None, _ptr(w2b), _ptr(net.b2.detach()), _ptr(h2), s), h, "conv2")
+    nat.check(L.sk_conv32(h, B, H, W, _ptr(h2), None, _ptr(w3b), _ptr(net.b3.detach()), _ptr(h3), s), h, "conv3")
+    nchw = lambda t: t.float().permute(0, 3, 1, 2).contiguous().cpu()
+    return nchw(h1), nchw(h2), nchw(h3)
+
+
+def _lin_grads(conv, inp, w, b, gout):
+    """FP64 gradients of a linear conv layer: (d input, d weight, d bias)."""
+    ins = [t.detach().double().clone().requires_grad_(True) for t in (inp, w, b)]
+    out = conv(*ins)
+    return torch.autograd.grad(out, ins, gout.double())
 
 
 def test_backward_matches_cpu_reference(ctx):
     """Gradients w.r.t. the input and every weight against FP64 autograd of each layer, with
-    the kernels' BF16 rounding points (weights, activations, activation gradients)."""
+    the kernels' BF16 rounding points (weights, activations, activation gradients) and the
+    ReLU masks of the kernels' own forward activations (a mask recomputed on the CPU can flip
+    where a pre-activation is within rounding of zero)."""
     net = _net(ctx, seed=2)
-    x = torch.rand(2, 40, 36, device="cuda", requires_grad=True)
-    gy = torch.randn(2, 40, 36, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand(2, 40, 36, device="cuda", generator=gen).requires_grad_(True)
+    gy = torch.randn(2, 40, 36, device="cuda", generator=gen)
     y = net(x)
     grads = torch.autograd.grad(y, [x, net.w1, net.b1, net.w2, net.b2, net.w3, net.b3, net.w4, net.b4], gy)
     cpu = lambda t: t.detach().cpu()
     w1, w2, w3, w4 = (_bf(cpu(w)) for w in (net.w1, net.w2, net.w3, net.w4))
     b1, b2, b3, b4 = (cpu(b) for b in (net.b1, net.b2, net.b3, net.b4))
+    h1, h2, h3 = _gpu_activations(ctx, net, x.detach())
     xc = cpu(x).unsqueeze(1)
-    c1 = lambda inp, w, b: F.relu(F.conv2d(inp, w.unsqueeze(1).contiguous(), b, padding=1))
-    c32 = lambda inp, w, b: F.relu(F.conv2d(inp, w.permute(0, 3, 1, 2).contiguous(), b, padding=1))
+    c1 = lambda inp, w, b: F.conv2d(inp, w.unsqueeze(1).contiguous(), b, padding=1)
+    c32 = lambda inp, w, b: F.conv2d(inp, w.permute(0, 3, 1, 2).contiguous(), b, padding=1)
     c4 = lambda inp, w, b: F.conv2d(inp, w.permute(2, 0, 1).unsqueeze(0).contiguous(), b, padding=1)
-    h1 = _bf(c1(xc, w1, b1))
-    h2 = _bf(c32(h1, w2, b2))
-    h3 = _bf(c32(h2, w3, b3))
     g = cpu(gy).unsqueeze(1)
-    dh3, gw4, gb4 = _layer_grads(c4, [h3, w4, b4], g)
-    dh2, gw3, gb3 = _layer_grads(c32, [h2, w3, b3], _bf(dh3.float()))
-    dh1, gw2, gb2 = _layer_grads(c32, [h1, w2, b2], _bf(dh2.float()))
-    dx, gw1, gb1 = _layer_grads(c1, [xc, w1, b1], _bf(dh1.float()))
+    dh3, gw4, gb4 = _lin_grads(c4, h3, w4, b4, g)
+    dp3 = _bf(dh3.float()) * (h3 > 0)               # ReLU backward with the kernel's mask
+    dh2, gw3, gb3 = _lin_grads(c32, h2, w3, b3, dp3)
+    dp2 = _bf(dh2.float()) * (h2 > 0)
+    dh1, gw2, gb2 = _lin_grads(c32, h1, w2, b2, dp2)
+    dp1 = _bf(dh1.float()) * (h1 > 0)
+    dx, gw1, gb1 = _lin_grads(c1, xc, w1, b1, dp1)
     dx = dx + g.double()
     ref = [dx.squeeze(1), gw1, gb1, gw2, gb2, gw3, gb3, gw4, gb4]
     names = ["x", "w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4"]
     for n, gg, r in zip(names, grads, ref):
         err = rel_l2(gg.cpu().numpy(), r.numpy())
-        assert err < 2e-3, (n, err)
+        print(f"grad {n}: rel-L2 {err:.2e}")
+        assert err < 5e-4, (n, err)
 
 
 def test_conv32_is_mfma_gemm_exactly_on_bf16_inputs(ctx):

@@ -269,3 +269,32 @@ def test_fp64_twin_agrees_off_edge_aligned_angles():
     keep[45] = False
     assert rel_l2(f32[keep], f64[keep]) < 1e-6
     assert rel_l2(f32[45], f64[45]) > 1e-3
+
+
+# ---------------------------------------------------------------- comparison tools (a14)
+def test_tools_known_answers():
+    """tools/sanityCheck.fut, diff.fut, mse.fut on hand-made cases, including the
+    reference's quirks: diff's filter (> 0) drops index 0, and a difference of exactly the
+    tolerance neither passes sanityCheck (< 0.05) nor counts in diff (> 0.05)."""
+    z = np.zeros(6, np.float32)
+    assert oracle.sanity_check(z, z) and oracle.diff(z, z) == 0 and oracle.mse(z, z) == 0.0
+    a = z.copy()
+    a[0] = 1.0                       # index 0: fails sanity, not counted by diff
+    assert not oracle.sanity_check(a, z) and oracle.diff(a, z) == 0
+    a[2] = np.float32(0.05)          # exactly the tolerance: fails sanity, not counted
+    assert oracle.diff(a, z) == 0
+    a[4] = -3.0                      # counted
+    assert oracle.diff(a, z) == 1
+    assert oracle.mse(a, z) == np.float32((1.0 + np.float32(0.05) ** 2 + 9.0) / 6)
+    b = z.copy()
+    b[3] = np.nan                    # NaN: fails sanity, not counted by diff
+    assert not oracle.sanity_check(b, z) and oracle.diff(b, z) == 0
+    c = z.copy()
+    c[1:] = np.float32(0.049)        # inside the tolerance everywhere
+    assert oracle.sanity_check(c, z) and oracle.diff(c, z) == 0
+
+
+def test_tools_mse_closed_form():
+    n = 1000
+    d = np.arange(n, dtype=np.float32) % 7     # small integers: every square and sum exact
+    assert oracle.mse(d, np.zeros(n, np.float32)) == np.float32(np.sum(d.astype(np.float64) ** 2) / n)
