@@ -138,9 +138,9 @@ typedef struct {
  * whole sequence runs as one hipGraph.  Synchronizes `stream` before returning. */
 int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream);
 
-/* No reference counterpart: time the kernel variants (fp: 1, 2 or 4 rays per lane; bp: 16x16,
- * 16x32, 16x64 pixel tiles) on this geometry and keep the fastest for it.  Variants compute
- * identical results.  Synchronizes `stream`. */
+/* No reference counterpart: time the kernel variants (fp: 1, 2 or 4 rays per lane, bin blocks
+ * or lines fastest in the grid; bp: 16x16, 16x32, 16x64 pixel tiles) on this geometry and keep
+ * the fastest for it.  Variants compute identical results.  Synchronizes `stream`. */
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
 /* Apply previously tuned variants (-1 = the context default). */
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant);

@@ -5,6 +5,6 @@ Layers:
   projector                  contexts, geometries, operators on torch tensors
   futhark                    the reference's module/class/``main`` surface
 """
-from .projector import Context, Geometry, centred_bins, uniform_angles  # noqa: F401
+from .projector import Context, Geometry, Matrix, centred_bins, sirt_batched, uniform_angles  # noqa: F401
 
 __version__ = "0.1.0"

@@ -19,6 +19,20 @@ constexpr unsigned BUF_DWORD3 = 0x00020000u;  // gfx9 raw buffer descriptor word
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// floor(x) and floor(-x) = -ceil(x) as an int in one instruction (v_cvt_flr_i32_f32).  The
+// compiler only selects this instruction under fast-math flags, which would also license
+// contractions elsewhere; the arguments here are finite and far inside the int range.
+__device__ __forceinline__ int floor_to_int(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ int neg_ceil_to_int(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, -%1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // A buffer descriptor covering one image row of n floats: loads at columns outside [0, n)
 // return 0 (hardware range check), which is exactly the reference's contribution of an
 // out-of-bounds pixel.
@@ -166,8 +180,8 @@ __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int
     for (int k = 0; k < RPT; ++k) {
       xbo[k] = yslope + icpt[k];
       xto[k] = xbo[k] + slope;
-      vbo[k] = row_load(row, (int)floorf(xbo[k]) + h);
-      vto[k] = row_load(row, (int)floorf(xto[k]) + h);
+      vbo[k] = row_load(row, floor_to_int(xbo[k]) + h);
+      vto[k] = row_load(row, floor_to_int(xto[k]) + h);
     }
   };
   auto consume = [&](const float* xbi, const float* xti, const float* vbi, const float* vti) {
@@ -351,6 +365,9 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
                                         float2* win, BpLineLds* lsh) {
   // Fast reference weights: slope sign hoisted per line, lbase folded into the staged value.
   constexpr bool FASTREF = !EXACT && WEIGHTS == W_REF;
+  // The hoisted fast path keeps each line's window in descending bin order, so that the
+  // candidate address comes from floor(-t) (one conversion) instead of ceil(t).
+  constexpr bool REV = FASTREF && NC > 0;
   const int ncand = NC > 0 ? NC : P.ncand;
   const int W = P.W;
   const int tx = threadIdx.x & (BP_TW - 1), ty = threadIdx.x / BP_TW;
@@ -407,7 +424,7 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         float p = 0.0f;
         if (sp >= 0 && sp < P.R) p = (MODE == BP_ONES_INV) ? 1.0f : srow[sp];
         if constexpr (FASTREF) p = p * L.lbase;
-        win[jj * W + k] = make_float2(inter, p);
+        win[jj * W + (REV ? W - 1 - k : k)] = make_float2(inter, p);
       }
     }
     __syncthreads();
@@ -437,6 +454,63 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
         // window entry of absolute bin s: wline[s] (w0 is uniform: scalar base arithmetic)
         const float2* wline = win + __builtin_amdgcn_readfirstlane(jj * W - lc.w0);
+        if constexpr (REV) {
+          // All of the lane's pixels first compute their candidate index and read their
+          // candidates, then one branch per line (degenerate / slope sign) covers every
+          // pixel: the LDS reads of all pixels are in flight together.  Bin s sits at
+          // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of
+          // the pixel whose first candidate is s.
+          const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
+          float2 ev[PPT][NC];
+          float xbase[PPT];
+#pragma unroll
+          for (int q = 0; q < PPT; ++q) {
+            const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
+            const float2* wrow = wrev + neg_ceil_to_int(__builtin_amdgcn_fmed3f(t, lc.lo, lc.hi));
+            xbase[q] = ym[q] * L.slope;  // backprojection.fut:55
+#pragma unroll
+            for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
+          }
+          if (!degen) {
+            // max(min(a, b), 0) == med3(a, b, 0): a and b cannot both be negative when the
+            // ray's x-range has positive length; m/diff is then >= 0, so min(., 1) is a
+            // clamp to [0, 1].
+            if (pos) {
+#pragma unroll
+              for (int q = 0; q < PPT; ++q)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                  const float xb = xbase[q] + ev[q][k].x;
+                  const float xt = xb + L.slope;
+                  const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
+                  const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
+                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+                }
+            } else {
+#pragma unroll
+              for (int q = 0; q < PPT; ++q)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                  const float xb = xbase[q] + ev[q][k].x;
+                  const float xt = xb + L.slope;
+                  const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                  const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
+                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+                }
+            }
+          } else {
+            // degenerate line: general form with the m == 0 case, lbase already in e.y
+#pragma unroll
+            for (int q = 0; q < PPT; ++q)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                const float l = cand_len<false, W_REF>(xbase[q] + ev[q][k].x, L.slope, 1.0f, xm[q],
+                                                       x1[q], true);
+                acc[q] = fmaf(l, ev[q][k].y, acc[q]);
+              }
+          }
+          continue;
+        }
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           const float xmin = xm[q], ymin = ym[q], xmin1 = x1[q];

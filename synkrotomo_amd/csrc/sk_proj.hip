@@ -43,11 +43,16 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
   }
 }
 
-// Grid: (bin blocks, lines, images).
-template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
+// Grid: (bin blocks, lines, images), or with LINES_FAST (lines, bin blocks, images): then
+// consecutive workgroups -- dispatched together, round-robin over the XCDs -- are the same
+// bins of neighbouring angles, whose rays cross nearly the same pixels at the same time.
+template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT, bool LINES_FAST = false>
 __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.y, blockIdx.x * FP_WAVES + wave, blockIdx.z);
+  if constexpr (LINES_FAST)
+    fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.x, blockIdx.y * FP_WAVES + wave, blockIdx.z);
+  else
+    fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.y, blockIdx.x * FP_WAVES + wave, blockIdx.z);
 }
 
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
@@ -76,14 +81,19 @@ FpParams fp_params(const Geom* g, const FpArgs& a) {
                   a.img, a.imgT, a.out, a.b, a.rowinv};
 }
 
-int fp_rays_per_lane(int variant) { return variant == 1 ? 2 : (variant == 2 ? 4 : 1); }
+// fp variants: 0/1/2 = 1/2/4 rays per lane, bin blocks fastest; 3/4 = 1/2 rays per lane,
+// lines fastest.
+int fp_rays_per_lane(int variant) {
+  return (variant == 1 || variant == 4) ? 2 : (variant == 2 ? 4 : 1);
+}
 int bp_pixels_per_lane(int variant) { return variant == 1 ? 2 : (variant == 2 ? 4 : 1); }
 
-template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT>
+template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT, bool LINES_FAST = false>
 static void fp_go(const Geom* g, const FpArgs& a, hipStream_t s) {
   const int per_block = FP_BLOCK * RPT;
-  dim3 grid((g->R + per_block - 1) / per_block, g->A, a.batch);
-  hipLaunchKernelGGL((k_fp<EXACT, MODE, WEIGHTS, DR1, RPT>), grid, dim3(FP_BLOCK), 0, s,
+  const unsigned nb = (unsigned)((g->R + per_block - 1) / per_block);
+  const dim3 grid = LINES_FAST ? dim3(g->A, nb, a.batch) : dim3(nb, g->A, a.batch);
+  hipLaunchKernelGGL((k_fp<EXACT, MODE, WEIGHTS, DR1, RPT, LINES_FAST>), grid, dim3(FP_BLOCK), 0, s,
                      fp_params(g, a));
 }
 
@@ -93,6 +103,10 @@ static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) 
     fp_go<EXACT, MODE, W_REF, true, 2>(g, a, s);
   else if (!EXACT && variant == 2)
     fp_go<EXACT, MODE, W_REF, true, 4>(g, a, s);
+  else if (!EXACT && variant == 3)
+    fp_go<EXACT, MODE, W_REF, true, 1, true>(g, a, s);
+  else if (!EXACT && variant == 4)
+    fp_go<EXACT, MODE, W_REF, true, 2, true>(g, a, s);
   else
     fp_go<EXACT, MODE, W_REF, true, 1>(g, a, s);
 }
