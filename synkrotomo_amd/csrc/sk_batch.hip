@@ -65,7 +65,8 @@ __device__ __forceinline__ int find_prob(const BatchArgs& a, long item) {
 // CLAIM consecutive items at a time from a global counter (items vary in cost -- rays near
 // the detector edge are clipped short -- and the counter balances them); otherwise items
 // are strided statically over the grid's waves.  No barriers either way.  Consecutive
-// items are adjacent bins of one line, so concurrent gathers share image rows.
+// items are the same bins of neighbouring lines (angles), whose rays cross nearly the same
+// pixels at the same time, so concurrent gathers share cache lines.
 constexpr int CLAIM = 2;
 
 template <bool EXACT, int RPT, bool DYN>
@@ -92,8 +93,8 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchArgs a, unsi
     const long per_img = (long)B.fp.A * B.fp_wb;
     const int bz = (int)(local / per_img);
     const long rem = local - (long)bz * per_img;
-    const int j = (int)(rem / B.fp_wb);
-    const int wb = (int)(rem - (long)j * B.fp_wb);
+    const int wb = (int)(rem / B.fp.A);
+    const int j = (int)(rem - (long)wb * B.fp.A);
     fp_item<EXACT, FP_SIRT, W_REF, true, RPT>(B.fp, u32(j), u32(wb), u32(bz));
     item += DYN ? 1 : nwaves;
   }
@@ -287,7 +288,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
 
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
   if (!g) return SK_EINVAL;
-  if (fp_variant < -1 || fp_variant > 4 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
+  if (fp_variant < -1 || fp_variant > 6 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
   g->g.fp_variant = fp_variant;
   g->g.bp_variant = bp_variant;
   return SK_OK;
@@ -327,7 +328,7 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
   };
   const bool verbose = getenv("SYNKRO_VERBOSE") != nullptr;
   float best = 1e30f, ms = 0.0f;
-  for (int v = 0; v <= 4 && !rc; v++) {
+  for (int v = 0; v <= 6 && !rc; v++) {
     rc = time_it(true, v, &ms);
     if (!rc && ms < best) { best = ms; best_fp = v; }
     if (!rc && verbose) fprintf(stderr, "sk_autotune: fp variant %d: %.3f ms / 3\n", v, ms);

@@ -98,6 +98,10 @@ EDGE = [
     ("drho_half", 24, _uniform(19), 70, -17.25, 0.5),
     ("drho_two", 40, _uniform(23), 30, -29.0, 2.0),
     ("axis_edges", 64, np.array([0.0, 1.5707964, 0.7853982, 2.3561945], np.float32), 91, -45.0, 1.0),
+    # every ray misses the image: all-zero outputs through the far end of the index ranges
+    ("far_miss", 32, _uniform(9), 40, 1e5, 1.0),
+    # a long detector: bin indices near 5e4 and a 100,003-bin sinogram row
+    ("long_detector", 32, _uniform(7), 100003, -50001.0, 1.0),
 ]
 
 
