@@ -44,29 +44,18 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in,
 }
 
 // Workgroup -> (line, bin block) orders.  ORDER 0: grid (bin blocks, lines, images).
-// ORDER 1: grid (lines, bin blocks, images): consecutive workgroups -- dispatched together --
-// are the same bins of neighbouring angles, whose rays cross nearly the same pixels at the
-// same time.  ORDER 2: as 1, but workgroups are dispatched round-robin over the 8 XCDs (each
-// with its own L2), so the linear workgroup id is permuted to give every XCD one contiguous
-// run of (line, bin block) items: neighbouring angles then share an L2.
-constexpr int NUM_XCD = 8;
+// ORDER 1: grid (lines, bin blocks, images): the workgroups resident together are one bin
+// block of (nearly) every angle -- one ring of the image at a time, with each row touched by
+// the rays of many neighbouring angles while it is cached.  (Permuting the lines so that
+// each XCD gets a contiguous run of angles measured 1.2x slower, and giving each XCD whole
+// bin blocks 1.6x slower, from the imbalance between short edge rays and long central ones.)
 template <bool EXACT, int MODE, int WEIGHTS, bool DR1, int RPT, int ORDER = 0>
 __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if constexpr (ORDER == 0) {
+  if constexpr (ORDER == 0)
     fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.y, blockIdx.x * FP_WAVES + wave, blockIdx.z);
-  } else if constexpr (ORDER == 1) {
+  else
     fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.x, blockIdx.y * FP_WAVES + wave, blockIdx.z);
-  } else {
-    const unsigned G = gridDim.x * gridDim.y;               // workgroups per image
-    const unsigned b = blockIdx.x + gridDim.x * blockIdx.y;  // dispatch order within an image
-    const unsigned per = (G + NUM_XCD - 1) / NUM_XCD;
-    const unsigned tall = G % NUM_XCD == 0 ? NUM_XCD : G % NUM_XCD;  // XCDs with `per` items
-    const unsigned x = b % NUM_XCD, k = b / NUM_XCD;
-    const unsigned item = x < tall ? x * per + k : tall * per + (x - tall) * (per - 1) + k;
-    const unsigned j = item % gridDim.x, bb = item / gridDim.x;
-    fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, j, bb * FP_WAVES + wave, blockIdx.z);
-  }
 }
 
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
@@ -96,9 +85,9 @@ FpParams fp_params(const Geom* g, const FpArgs& a) {
 }
 
 // fp variants: 0/1/2 = 1/2/4 rays per lane, bin blocks fastest; 3/4 = 1/2 rays per lane,
-// lines fastest; 5/6 = 1/2 rays per lane, lines fastest in XCD-contiguous runs.
+// lines fastest.
 int fp_rays_per_lane(int variant) {
-  return (variant == 1 || variant == 4 || variant == 6) ? 2 : (variant == 2 ? 4 : 1);
+  return (variant == 1 || variant == 4) ? 2 : (variant == 2 ? 4 : 1);
 }
 int bp_pixels_per_lane(int variant) { return variant == 1 ? 2 : (variant == 2 ? 4 : 1); }
 
@@ -121,10 +110,6 @@ static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) 
     fp_go<EXACT, MODE, W_REF, true, 1, 1>(g, a, s);
   else if (!EXACT && variant == 4)
     fp_go<EXACT, MODE, W_REF, true, 2, 1>(g, a, s);
-  else if (!EXACT && variant == 5)
-    fp_go<EXACT, MODE, W_REF, true, 1, 2>(g, a, s);
-  else if (!EXACT && variant == 6)
-    fp_go<EXACT, MODE, W_REF, true, 2, 2>(g, a, s);
   else
     fp_go<EXACT, MODE, W_REF, true, 1>(g, a, s);
 }

@@ -22,7 +22,7 @@ def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
     y = torch.rand(2, A, R, device="cuda")
     geo.set_variants(0, 0)
     f0, b0 = geo.fp(x), geo.bp(y)
-    for fv in range(7):
+    for fv in range(5):
         for bv in (0, 1, 2):
             geo.set_variants(fv, bv)
             torch.testing.assert_close(geo.fp(x), f0, rtol=0, atol=0)
@@ -38,7 +38,7 @@ def test_autotune_cache(ctx, tmp_path, monkeypatch):
     monkeypatch.setenv("SYNKRO_TUNE_CACHE", str(tmp_path / "tune.json"))
     geo = ctx.geometry(_uniform(180), -181.0, 1.0, 363, 256)
     fv, bv = autotune.tuned(geo)
-    assert fv in range(7) and bv in (0, 1, 2)
+    assert fv in range(5) and bv in (0, 1, 2)
     assert (tmp_path / "tune.json").exists()
     assert autotune.tuned(geo) == (fv, bv)  # served from the cache
     geo.set_variants(-1, -1)
