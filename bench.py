@@ -149,8 +149,41 @@ def learned_leg(ctx, n_test: int = 32) -> dict:
     from synkrotomo_amd import learned_run
     if not os.path.exists(learned_run.DEFAULT_WEIGHTS):
         return {"skipped": "no trained weights"}
-    return learned_run.evaluate(ctx, learned_run.DEFAULT_WEIGHTS, n_test, batch=16, max_iters=200,
-                                verbose=False)
+    out = learned_run.evaluate(ctx, learned_run.DEFAULT_WEIGHTS, n_test, batch=16, max_iters=200,
+                               verbose=False)
+    out["conv32"] = conv32_rate(ctx)
+    return out
+
+
+def conv32_rate(ctx, B: int = 16, H: int = 512, W: int = 512, reps: int = 20) -> dict:
+    """The 32->32 3x3 BF16 MFMA convolution (k_conv32, bias + ReLU epilogue) on a batch of 16
+    512^2 x 32-channel activations: achieved TFLOP/s (2*9*32*32 FLOP per output pixel) against
+    the 2.5 PFLOP/s dense BF16 MFMA peak, and the activation bytes it streams against HBM."""
+    import torch
+    from synkrotomo_amd import _native as nat
+    from synkrotomo_amd.projector import _ptr, _stream
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.rand(B, H, W, 32, device="cuda", generator=g).to(torch.bfloat16)
+    w = ((torch.rand(32, 3, 3, 32, device="cuda", generator=g) - 0.5) * 0.1).to(torch.bfloat16)
+    b = torch.zeros(32, device="cuda")
+    y = torch.empty_like(x)
+    L, h, s = nat.lib(), ctx.handle, _stream(ctx.device)
+    run = lambda: nat.check(L.sk_conv32(h, B, H, W, _ptr(x), None, _ptr(w), _ptr(b), _ptr(y), s), h, "conv32")
+    for _ in range(3):
+        run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = 2.0 * 9 * 32 * 32 * B * H * W
+    act_bytes = 2.0 * 2 * 32 * B * H * W  # bf16 activations in and out
+    return {"shape": f"{B}x{H}x{W}x32 bf16", "ms": round(ms, 4),
+            "tflops": round(flops / (ms * 1e-3) / 1e12, 1),
+            "mfma_peak_tflops": 2500.0, "mfma_frac": round(flops / (ms * 1e-3) / 2.5e15, 4),
+            "activation_GBs": round(act_bytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def main() -> int:

@@ -21,9 +21,13 @@ def cache_path() -> str:
                           os.path.join(os.path.expanduser("~"), ".cache", "synkrotomo_amd", "tune.json"))
 
 
+# Bumped whenever the meaning of the variant numbers changes, so old cache entries are ignored.
+VARIANT_TABLE = 2
+
+
 def _key(geom: Geometry) -> str:
     arch = torch.cuda.get_device_properties(geom.ctx.device).gcnArchName.split(":")[0]
-    return f"{arch}/N{geom.N}/A{geom.A}/R{geom.R}/d{geom.drho:g}"
+    return f"v{VARIANT_TABLE}/{arch}/N{geom.N}/A{geom.A}/R{geom.R}/d{geom.drho:g}"
 
 
 def _load() -> dict:
@@ -52,8 +56,11 @@ def tuned(geom: Geometry, retune: bool = False) -> tuple[int, int]:
     key = _key(geom)
     if not retune and key in table:
         fv, bv = table[key]
-        geom.set_variants(fv, bv)
-        return fv, bv
+        try:
+            geom.set_variants(fv, bv)
+            return fv, bv
+        except ValueError:
+            pass  # an entry the library no longer accepts: measure again
     fv, bv = geom.autotune()
     table[key] = [fv, bv]
     _store(table)

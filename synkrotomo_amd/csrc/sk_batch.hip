@@ -229,7 +229,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     args.fp_total += (long)P.batch * G.A * B.fp_wb;
     args.bp_total += (long)P.batch * B.bp_tx * B.bp_ty;
     plan.Wmax = std::max(plan.Wmax, B.bp.W);
-    plan.dr1 = plan.dr1 && G.drho == 1.0f && G.ncand == 2;
+    plan.dr1 = plan.dr1 && G.bp_two_cand;
     args.p[oi] = B;
     rc = launch_transpose(c, P.d_img, imgT, G.N, P.batch, s);
     if (rc) return rc;

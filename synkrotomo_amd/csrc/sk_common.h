@@ -32,6 +32,9 @@ struct alignas(32) Line {
 struct Geom {
   int A = 0, R = 0, N = 0, n_steep = 0, ncand = 0;
   float rho0 = 0.f, drho = 1.f;
+  // Unit bins with two candidates per pixel and |rho0| + N < 2^20, where the fast candidate
+  // index (rounding error < 0.4 bin) needs no clamp: the specialised backprojector applies.
+  bool bp_two_cand = false;
   std::vector<Line> lines;   // host copy, steep lines then flat lines
   Line* d_lines = nullptr;
   // SIRT weights, computed on first use (SIRT.fut:22-25)

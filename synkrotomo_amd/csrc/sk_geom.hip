@@ -174,6 +174,7 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   sk_geom* g = new sk_geom();
   Geom& G = g->g;
   G.A = A; G.R = R; G.N = N; G.rho0 = rho0; G.drho = drho; G.ncand = ncand;
+  G.bp_two_cand = drho == 1.0f && ncand == 2 && std::fabs((double)rho0) + N < (double)(1 << 20);
   G.device = c->device;
   G.n_steep = build_lines(angles_host, A, G.lines);
   hipError_t e = hipMalloc(&G.d_lines, sizeof(Line) * (size_t)A);

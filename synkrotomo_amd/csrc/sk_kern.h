@@ -338,13 +338,11 @@ __host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
 // Per-line values of one line block, in LDS.  w0: the tile's rho-window origin.  Fast mode
 // only: ci, si, cst give the candidate index t = ci*(xmin+0.5) + si*(ymin+0.5) + cst (the
 // reference's ((cost*x + sint*y) - rho0)/drho, x = xmin+0.5-fact*cost, y = ymin+0.5-fact*sint,
-// with the constant terms folded), [lo, hi] the window's candidate range as floats.  t does
-// not depend on the tile, so every tile shape selects the same candidates.
+// with the constant terms folded).  t does not depend on the tile, so every tile shape
+// selects the same candidates.
 struct alignas(16) BpLineLds {
   int w0;
   float ci, si, cst;
-  float lo, hi;
-  int pad0, pad1;
 };
 
 // One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
@@ -398,11 +396,10 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
       smin = fminf(smin, cand_s<DR1>(L, cx0, cy1, P.rho0, P.drho));
       smin = fminf(smin, cand_s<DR1>(L, cx1, cy1, P.rho0, P.drho));
       BpLineLds e;
-      e.w0 = (int)smin;
+      // The hoisted fast path starts the window one bin lower: its index may round one bin
+      // below the reference's at the tile's lowest corner (the top keeps a one-bin margin).
+      e.w0 = (int)smin - (REV ? 1 : 0);
       e.ci = e.si = e.cst = 0.0f;
-      e.lo = smin;
-      e.hi = smin + (float)(W - ncand);
-      e.pad0 = e.pad1 = 0;
       if constexpr (FASTREF) {
         const float idr = DR1 ? 1.0f : 1.0f / P.drho;
         e.ci = L.cost * idr;
@@ -452,21 +449,21 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         const bool pos = L.flags & LINE_SLOPE_POS;
         // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
         const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
-        // window entry of absolute bin s: wline[s] (w0 is uniform: scalar base arithmetic)
-        const float2* wline = win + __builtin_amdgcn_readfirstlane(jj * W - lc.w0);
         if constexpr (REV) {
           // All of the lane's pixels first compute their candidate index and read their
           // candidates, then one branch per line (degenerate / slope sign) covers every
           // pixel: the LDS reads of all pixels are in flight together.  Bin s sits at
           // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of
-          // the pixel whose first candidate is s.
+          // the pixel whose first candidate is s (w0 is uniform: scalar base arithmetic).
           const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
           float2 ev[PPT][NC];
           float xbase[PPT];
 #pragma unroll
           for (int q = 0; q < PPT; ++q) {
+            // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this path),
+            // so its ceiling lies in [w0, w0 + W - NC]: no clamp.
             const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
-            const float2* wrow = wrev + neg_ceil_to_int(__builtin_amdgcn_fmed3f(t, lc.lo, lc.hi));
+            const float2* wrow = wrev + neg_ceil_to_int(t);
             xbase[q] = ym[q] * L.slope;  // backprojection.fut:55
 #pragma unroll
             for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
@@ -517,8 +514,9 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
           const float xbase = ymin * L.slope;  // backprojection.fut:55
           const float2* wrow;
           if constexpr (FASTREF) {
+            // general path (any geometry): the fast index, clamped into the window
             const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
-            wrow = wline + (int)ceilf(__builtin_amdgcn_fmed3f(t, lc.lo, lc.hi));
+            wrow = win + jj * W + min(max((int)ceilf(t) - lc.w0, 0), W - ncand);
           } else {
             // backprojection.fut:51-54 with (xmin + 0.5), (ymin + 0.5) hoisted
             const float px = xh[q] - L.fc;

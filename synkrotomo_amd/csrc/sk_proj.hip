@@ -153,7 +153,7 @@ static void bp_go(const Geom* g, const BpArgs& a, hipStream_t s) {
 
 template <bool EXACT, int MODE, int WEIGHTS, int PPT>
 static void bp_go_nc(const Geom* g, const BpArgs& a, hipStream_t s) {
-  if (g->drho == 1.0f && g->ncand == 2)
+  if (g->bp_two_cand)
     bp_go<EXACT, MODE, WEIGHTS, 2, true, PPT>(g, a, s);
   else
     bp_go<EXACT, MODE, WEIGHTS, 0, false, PPT>(g, a, s);

@@ -44,8 +44,13 @@ DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "learned_sirt_512_360.npz")
 
 
 def geometry(ctx):
+    """The 512^2 / 360-angle / 725-bin geometry with its autotuned kernel shapes (plain and
+    learned SIRT run the same projector kernels)."""
+    from .autotune import tuned
     R, rz = centred_bins(N)
-    return ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
+    geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
+    tuned(geo)
+    return geo
 
 
 def attenuation_scale(ctx, geo, rng) -> float:
@@ -62,7 +67,7 @@ def train(args) -> int:
     scale = attenuation_scale(ctx, geo, np.random.default_rng(TRAIN_SEED + 1))
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED)
     net = Correction(ctx, seed=0)
-    model = LearnedSIRT(geo, net, STAGES)
+    model = LearnedSIRT(geo, net, args.stages)
     steps = args.epochs * (args.phantoms // args.batch)
     opt = torch.optim.Adam(net.parameters(), lr=args.lr)
     sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=args.lr, total_steps=steps, pct_start=0.05)
@@ -105,7 +110,7 @@ def train(args) -> int:
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
-    net.save(args.out, scale=scale, I0=I0, stages=STAGES, N=N, A=A, phantoms=args.phantoms,
+    net.save(args.out, scale=scale, I0=I0, stages=args.stages, N=N, A=A, phantoms=args.phantoms,
              batch=args.batch, epochs=args.epochs, res_weight=args.res_weight,
              stage_weight=args.stage_weight, res_excess_weight=args.res_excess_weight,
              train_seconds=train_s)
@@ -126,7 +131,8 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
     net = Correction(ctx)
     meta = net.load(weights)
     scale = float(meta["scale"])
-    model = LearnedSIRT(geo, net, STAGES)
+    stages = int(meta["stages"]) if "stages" in meta else STAGES
+    model = LearnedSIRT(geo, net, stages)
     rng = np.random.default_rng(TEST_SEED)
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED + 1)
     # timing of one SIRT iteration and one stage (SIRT step + correction), batch of images
@@ -174,7 +180,7 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
             res_l = [residual(geo, xl, b).cpu().numpy()]
             err_l = [((xl - x_true).reshape(B, -1).double().norm(dim=1) /
                       x_true.reshape(B, -1).double().norm(dim=1)).cpu().numpy()]
-            for k in range(max_iters - STAGES):
+            for k in range(max_iters - stages):
                 geo.sirt_(xl, b, 1)
                 res_l.append(residual(geo, xl, b).cpu().numpy())
                 err_l.append(((xl - x_true).reshape(B, -1).double().norm(dim=1) /
@@ -192,7 +198,7 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                              "plain_res_at_max": float(res_p[-1, i]),
                              "learned_res_after_stages": float(res_l[0, i]),
                              "learned_err_after_stages": float(err_l[0, i]),
-                             "plain_err_at_5": float(err_p[STAGES - 1, i])})
+                             "plain_err_at_stages": float(err_p[stages - 1, i])})
         if verbose:
             print(json.dumps({"done": start + B}), flush=True)
 
@@ -208,7 +214,7 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         if both:
             tp = np.array([r["plain_iters"] * t_iter for r in both])
             tl = np.array([t_recon + r["learned_extra_iters"] * t_iter for r in both])
-            ts = np.array([STAGES * t_stage + r["learned_extra_iters"] * t_iter for r in both])
+            ts = np.array([stages * t_stage + r["learned_extra_iters"] * t_iter for r in both])
             summary.update({
                 "median_plain_ms": round(float(np.median(tp)) * 1e3, 3),
                 "median_learned_ms": round(float(np.median(tl)) * 1e3, 3),
@@ -222,8 +228,9 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         out[tname] = summary
     out["fell_back"] = sum(r["fell_back"] for r in rows if r["target"] == "1e-2")
     rs = [r for r in rows if r["target"] == "1e-2"]
-    out["after_5"] = {"median_learned_err": round(float(np.median([r["learned_err_after_stages"] for r in rs])), 5),
-                      "median_plain_err": round(float(np.median([r["plain_err_at_5"] for r in rs])), 5),
+    out["stages"] = stages
+    out["after_stages"] = {"median_learned_err": round(float(np.median([r["learned_err_after_stages"] for r in rs])), 5),
+                      "median_plain_err": round(float(np.median([r["plain_err_at_stages"] for r in rs])), 5),
                       "median_learned_res": round(float(np.median([r["learned_res_after_stages"] for r in rs])), 5)}
     return out
 
@@ -236,6 +243,7 @@ def main(argv=None) -> int:
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--epochs", type=int, default=3)
+    t.add_argument("--stages", type=int, default=STAGES, help="SIRT step + correction stages")
     t.add_argument("--res-weight", type=float, default=0.0,
                    help="weight of the final data residual ||Ax-b||^2/||b||^2 in the loss")
     t.add_argument("--stage-weight", type=float, default=0.0,
