@@ -88,17 +88,14 @@ def sparse_leg(geo, x, sino, K: int) -> dict:
     return res
 
 
-def batched_leg(ctx, iters: int = 30) -> dict:
-    """Batched SIRT: 64 x 256^2 (180 angles), 96 x 512^2 (360), 32 x 1024^2 (720) and a
-    SIRT3D stack of 512 slices of 512^2 (512 angles), centred unit bins, 30 iterations, in
-    one persistent launch sequence (one hipGraph).  Timed against the same problems run
-    group by group through sk_sirt with each geometry's autotuned kernel shapes.  Each path
-    runs twice and the second (warm) run is timed.  Sinograms are uniform random (timing is
-    data-independent)."""
+def _batched_case(ctx, specs, iters: int, seed: int) -> dict:
+    """specs: [(N, angles, images)], centred unit bins.  Batched SIRT (one persistent launch
+    sequence, one hipGraph) against the same problems group by group through sk_sirt with
+    each geometry's autotuned kernel shapes; each path runs twice and the second (warm) run
+    is timed.  Sinograms are uniform random (timing is data-independent)."""
     import torch
     from synkrotomo_amd.projector import centred_bins, sirt_batched, uniform_angles
-    specs = [(1024, 720, 32), (512, 360, 96), (256, 180, 64), (512, 512, 512)]
-    gen = torch.Generator(device="cuda").manual_seed(30)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
     groups = []
     updates = 0.0
     for n, a, cnt in specs:
@@ -136,9 +133,23 @@ def batched_leg(ctx, iters: int = 30) -> dict:
     same = all(bool(torch.equal(a[1], b)) for a, b in zip(groups, singles))
     del groups, singles
     torch.cuda.empty_cache()
-    return {"problems": "64x256^2/180 + 96x512^2/360 + 32x1024^2/720 + SIRT3D 512x512^2/512",
-            "iterations": iters, "batched_s": round(t_batched, 3), "per_group_s": round(t_groups, 3),
+    return {"iterations": iters, "batched_s": round(t_batched, 4), "per_group_s": round(t_groups, 4),
             "batched_gups": round(updates / t_batched / 1e9, 1), "identical_to_per_group": same}
+
+
+def batched_leg(ctx, iters: int = 30) -> dict:
+    """Batched SIRT on two problem sets: large groups (64 x 256^2 / 180 angles, 96 x 512^2 /
+    360, 32 x 1024^2 / 720 and a SIRT3D stack of 512 slices of 512^2 / 512), and 16 small
+    groups of 4 images each (N = 64, 96, ..., 544 with 0.7 N angles), where one geometry alone
+    does not fill the GPU."""
+    big = _batched_case(ctx, [(1024, 720, 32), (512, 360, 96), (256, 180, 64), (512, 512, 512)],
+                        iters, 30)
+    big["problems"] = "64x256^2/180 + 96x512^2/360 + 32x1024^2/720 + SIRT3D 512x512^2/512"
+    small = _batched_case(ctx, [(n, int(0.7 * n), 4) for n in range(64, 545, 32)], iters, 31)
+    small["problems"] = "16 groups x 4 images, N = 64..544 step 32, 0.7N angles"
+    out = dict(big)
+    out["small_groups"] = small
+    return out
 
 
 def learned_leg(ctx, n_test: int = 32) -> dict:

@@ -210,9 +210,8 @@ int sk_mat_arrays(const sk_mat* m, const int64_t** d_rowptr, const int32_t** d_c
 int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols, float* d_vals,
                 void* stream);
 /* SpMV kernel: 0 = one wave per row, 1 = one lane per row (64 adjacent rows in lockstep),
- * 2 = one wave per row with 16-byte column/value streams, 3 = one lane per row reading its
- * entries in bursts of 16 (64 adjacent rows in lockstep); -1 (default) = time all on the
- * first sk_spmv call and keep the fastest. */
+ * 2 = one wave per row with 16-byte column/value streams; -1 (default) = time 0 and 2 on the
+ * first sk_spmv call and keep the faster. */
 int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 /* d_out[batch][rows] = M . d_in[batch][cols] */
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);

@@ -125,10 +125,10 @@ __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned*
 
 struct BatchPlan {
   BatchArgs args{};
-  // bit0: 2 rays per lane, bit1: dynamic claiming.  3 measured fastest on the batched
-  // configuration (static striding leaves waves with the costliest items idle-waiting at
-  // the end); SYNKRO_BATCH_FP_MODE overrides it.
-  int fp_mode = 3;
+  // bit0: 2 rays per lane, bit1: dynamic claiming.  2 measured fastest on the batched
+  // configuration (3.82 s vs 4.02 s for mode 3 and 5.10 s for mode 0: static striding leaves
+  // waves with the costliest items idle-waiting at the end); SYNKRO_BATCH_FP_MODE overrides it.
+  int fp_mode = 2;
   unsigned* d_ctr = nullptr;
   int Wmax = 0;
   bool dr1 = true;
@@ -242,7 +242,12 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   const bool ex = c->exact != 0;
   const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
   int fp_grid = 0, bp_grid = 0;
-  const void* kfp = ex ? (const void*)k_fp_persist<true, 2, true> : (const void*)k_fp_persist<false, 2, true>;
+  // occupancy (and so the persistent grid) of the forward kernel that will run
+  const void* kfp_ex[4] = {(const void*)k_fp_persist<true, 1, false>, (const void*)k_fp_persist<true, 2, false>,
+                           (const void*)k_fp_persist<true, 1, true>, (const void*)k_fp_persist<true, 2, true>};
+  const void* kfp_fast[4] = {(const void*)k_fp_persist<false, 1, false>, (const void*)k_fp_persist<false, 2, false>,
+                             (const void*)k_fp_persist<false, 1, true>, (const void*)k_fp_persist<false, 2, true>};
+  const void* kfp = ex ? kfp_ex[plan.fp_mode & 3] : kfp_fast[plan.fp_mode & 3];
   const void* kbp =
       plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, BATCH_PPT> : (const void*)k_bp_persist<false, 2, true, BATCH_PPT>)
                : (ex ? (const void*)k_bp_persist<true, 0, false, BATCH_PPT> : (const void*)k_bp_persist<false, 0, false, BATCH_PPT>);

@@ -33,7 +33,7 @@
 struct sk_mat {
   int kind = 0;
   int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
-                          // 16-byte streams, 3: lane per row in 16-entry bursts; -1: not timed
+                          // 16-byte streams; -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
@@ -354,44 +354,6 @@ static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr
   return SK_OK;
 }
 
-// Chunked lockstep SpMV: one lane per row as in k_spmv_lockstep, but each lane reads its
-// row's columns and values CH entries at a time in a burst of consecutive loads (its cache
-// line stays in L1 across the burst), then the wave gathers and accumulates the CH entries
-// in lockstep, so the gathers of one step still fall on neighbouring addresses.  Sums in
-// CSR order.
-template <int CH>
-__global__ __launch_bounds__(256) void k_spmv_chunked(const int64_t* __restrict__ rowptr,
-                                                      const int32_t* __restrict__ col,
-                                                      const float* __restrict__ val, long rows,
-                                                      long ncols, const float* __restrict__ x,
-                                                      float* __restrict__ y) {
-  const long row = (long)blockIdx.x * 256 + threadIdx.x;
-  const float* xb = x + (size_t)blockIdx.y * ncols;
-  long beg = 0, end = 0;
-  if (row < rows) {
-    beg = rowptr[row];
-    end = rowptr[row + 1];
-  }
-  const int len = (int)(end - beg);
-  int maxlen = len;
-  for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, off));
-  float acc = 0.0f;
-  for (int k = 0; k < maxlen; k += CH) {
-    int c[CH];
-    float v[CH];
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const bool ok = k + u < len;
-      c[u] = ok ? __builtin_nontemporal_load(col + beg + k + u) : 0;
-      v[u] = ok ? __builtin_nontemporal_load(val + beg + k + u) : 0.0f;
-    }
-#pragma unroll
-    for (int u = 0; u < CH; ++u)
-      if (k + u < len) acc = fmaf(v[u], xb[c[u]], acc);
-  }
-  if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
-}
-
 // Lockstep SpMV: one lane per row, the 64 rows of a wave advance through their entries
 // together.  Adjacent rows of A are rays of adjacent bins of one angle and adjacent rows
 // of B are neighbouring pixels, so at each step the wave's x gathers fall on neighbouring
@@ -555,7 +517,7 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 3) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 2) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -604,11 +566,7 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   auto run = [&](int v) {
-    if (v == 3) {
-      dim3 grid((unsigned)((m->rows + 255) / 256), batch);
-      hipLaunchKernelGGL(k_spmv_chunked<16>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
-                         (long)m->rows, (long)m->cols, d_in, d_out);
-    } else if (v == 2) {
+    if (v == 2) {
       dim3 grid((unsigned)((m->rows + 3) / 4), batch);
       hipLaunchKernelGGL(k_spmv_wide<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
                          (long)m->rows, (long)m->cols, d_in, d_out);
@@ -623,32 +581,35 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     }
   };
   if (m->spmv_variant < 0) {
-    // First use: time the kernels once on this matrix and keep the fastest.
-    constexpr int NV = 4;
+    // First use: time the wave-per-row kernels once on this matrix and keep the faster.
+    // (The lockstep kernel, variant 1, measured 6-9x slower on both matrices at 2048^2:
+    // each of its load instructions touches 64 rows' cache lines.  It is selectable only.)
+    constexpr int NV = 2;
+    const int cand[NV] = {0, 2};
     hipEvent_t e[NV + 1];
     for (auto& ev : e) SK_HIP(c, hipEventCreate(&ev));
     SK_HIP(c, hipEventRecord(e[0], s));
     for (int v = 0; v < NV; ++v) {
-      run(v);
+      run(cand[v]);
       SK_HIP(c, hipEventRecord(e[v + 1], s));
     }
     SK_HIP(c, hipEventSynchronize(e[NV]));
-    int best = 0;
+    int best = cand[0];
     float tbest = 0.0f;
     for (int v = 0; v < NV; ++v) {
       float t = 0.0f;
       SK_HIP(c, hipEventElapsedTime(&t, e[v], e[v + 1]));
-      if (getenv("SYNKRO_VERBOSE")) fprintf(stderr, "sk_spmv: variant %d: %.3f ms\n", v, t);
+      if (getenv("SYNKRO_VERBOSE")) fprintf(stderr, "sk_spmv: variant %d: %.3f ms\n", cand[v], t);
       if (v == 0 || t < tbest) {
         tbest = t;
-        best = v;
+        best = cand[v];
       }
     }
     for (auto& ev : e) (void)hipEventDestroy(ev);
     const_cast<sk_mat*>(m)->spmv_variant = best;
     // The kernels sum a row in different orders; leave the chosen one's result, so that
     // every call on this matrix returns the same bits.
-    if (best != NV - 1) run(best);
+    if (best != cand[NV - 1]) run(best);
     return launched(c, "SpMV kernel");
   }
   run(m->spmv_variant);
