@@ -100,14 +100,16 @@ static void fp_go(const Geom* g, const FpArgs& a, hipStream_t s) {
                      fp_params(g, a));
 }
 
+// Exact mode runs one ray per lane (its IEEE divisions make more rays per lane no faster) in
+// either grid order.
 template <bool EXACT, int MODE>
 static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) {
-  if (!EXACT && variant == 1)
+  if (variant == 3 || (EXACT && variant == 4))
+    fp_go<EXACT, MODE, W_REF, true, 1, 1>(g, a, s);
+  else if (!EXACT && variant == 1)
     fp_go<EXACT, MODE, W_REF, true, 2>(g, a, s);
   else if (!EXACT && variant == 2)
     fp_go<EXACT, MODE, W_REF, true, 4>(g, a, s);
-  else if (!EXACT && variant == 3)
-    fp_go<EXACT, MODE, W_REF, true, 1, 1>(g, a, s);
   else if (!EXACT && variant == 4)
     fp_go<EXACT, MODE, W_REF, true, 2, 1>(g, a, s);
   else
@@ -161,7 +163,7 @@ static void bp_go_nc(const Geom* g, const BpArgs& a, hipStream_t s) {
 
 template <bool EXACT, int MODE, int WEIGHTS>
 static void bp_go_v(int variant, const Geom* g, const BpArgs& a, hipStream_t s) {
-  if (EXACT || WEIGHTS != W_REF || variant == 0)
+  if (WEIGHTS != W_REF || variant == 0)
     bp_go_nc<EXACT, MODE, WEIGHTS, 1>(g, a, s);
   else if (variant == 1)
     bp_go_nc<EXACT, MODE, WEIGHTS, 2>(g, a, s);

@@ -33,6 +33,21 @@ def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
         assert rel_l2(f0[0].cpu().numpy(), ref) <= 1e-5
 
 
+@pytest.mark.parametrize("N,A,R,rz,dr", [(64, 90, 91, -45.0, 1.0), (40, 23, 30, -29.0, 2.0)])
+def test_exact_mode_every_variant_bitwise_oracle(exact, N, A, R, rz, dr):
+    """Exact mode is bitwise the oracle with every kernel shape and grid order."""
+    rng = np.random.default_rng(8)
+    x = rng.random(N * N).astype(np.float32)
+    y = rng.random(A * R).astype(np.float32)
+    fref, bref = oracle.fp(_uniform(A), rz, dr, R, x), oracle.bp(_uniform(A), rz, dr, N, y)
+    geo = exact.geometry(_uniform(A), rz, dr, R, N)
+    for v in range(5):
+        geo.set_variants(v, min(v, 2))
+        np.testing.assert_array_equal(geo.fp(torch.from_numpy(x).cuda()).cpu().numpy().ravel(), fref)
+        np.testing.assert_array_equal(geo.bp(torch.from_numpy(y).cuda()).cpu().numpy().ravel(), bref)
+    geo.set_variants(-1, -1)
+
+
 def test_autotune_cache(ctx, tmp_path, monkeypatch):
     from synkrotomo_amd import autotune
     monkeypatch.setenv("SYNKRO_TUNE_CACHE", str(tmp_path / "tune.json"))
