@@ -138,9 +138,12 @@ typedef struct {
  * whole sequence runs as one hipGraph.  Synchronizes `stream` before returning. */
 int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream);
 
-/* No reference counterpart: time the kernel variants (fp: 1, 2 or 4 rays per lane, bin blocks
- * or lines fastest in the grid; bp: 16x16, 16x32, 16x64 pixel tiles) on this geometry and keep
- * the fastest for it.  Variants compute identical results.  Synchronizes `stream`. */
+/* No reference counterpart: time the kernel variants on this geometry and keep the fastest
+ * for it.  fp: 0/1/2 = a wave per 64/128/256 bins (1, 2 or 4 rays per lane), bin blocks
+ * fastest in the grid; 3/4 = 1 or 2 rays per lane, lines fastest; 5/6 = 256 bins per
+ * workgroup with each block of 16 image rows staged in LDS (double / single buffered; fast
+ * mode, deltarho up to about 1, otherwise they run variant 3).  bp: 16x16, 16x32, 16x64 pixel
+ * tiles.  Variants compute identical results.  Synchronizes `stream`. */
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
 /* Apply previously tuned variants (-1 = the context default). */
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant);

@@ -11,7 +11,8 @@
 // bp kernel the fp slot, so the whole iteration sequence (a reset kernel, then fp/bp pairs)
 // is captured once into a hipGraph and launched once.
 //
-// Autotuner: times the fp variants (1 or 2 rays per lane) and the bp variants (16x16,
+// Autotuner: times the fp variants (per-wave kernels with 1, 2 or 4 rays per lane, staged
+// kernels) and the bp variants (16x16,
 // 16x32 or 16x64 pixel tiles) on a geometry with HIP events and keeps the fastest.  All
 // variants evaluate the same expressions in the same order per ray and per pixel, so the
 // choice never changes results.
@@ -293,7 +294,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
 
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
   if (!g) return SK_EINVAL;
-  if (fp_variant < -1 || fp_variant > 4 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
+  if (fp_variant < -1 || fp_variant > 6 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
   g->g.fp_variant = fp_variant;
   g->g.bp_variant = bp_variant;
   return SK_OK;
@@ -333,7 +334,7 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
   };
   const bool verbose = getenv("SYNKRO_VERBOSE") != nullptr;
   float best = 1e30f, ms = 0.0f;
-  for (int v = 0; v <= 4 && !rc; v++) {
+  for (int v = 0; v <= 6 && !rc; v++) {
     rc = time_it(true, v, &ms);
     if (!rc && ms < best) { best = ms; best_fp = v; }
     if (!rc && verbose) fprintf(stderr, "sk_autotune: fp variant %d: %.3f ms / 3\n", v, ms);

@@ -35,6 +35,8 @@ struct Geom {
   // Unit bins with two candidates per pixel and |rho0| + N < 2^20, where the fast candidate
   // index (rounding error < 0.4 bin) needs no clamp: the specialised backprojector applies.
   bool bp_two_cand = false;
+  // Row-segment width (floats) the staged forward projector needs for 256 consecutive bins.
+  int fps_w = 0;
   std::vector<Line> lines;   // host copy, steep lines then flat lines
   Line* d_lines = nullptr;
   // SIRT weights, computed on first use (SIRT.fut:22-25)

@@ -177,6 +177,14 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   G.bp_two_cand = drho == 1.0f && ncand == 2 && std::fabs((double)rho0) + N < (double)(1 << 20);
   G.device = c->device;
   G.n_steep = build_lines(angles_host, A, G.lines);
+  {
+    // Staged forward projector: the intercepts of 256 consecutive bins span 255*drho/|cost|
+    // columns, plus their rounding; a row segment adds 4 columns (the pair and the floors).
+    double inv = 0.0;
+    for (const Line& L : G.lines) inv = std::max(inv, 1.0 / std::fabs((double)L.cost));
+    const double span = 255.0 * drho * inv + 4.0 * (std::fabs((double)rho0) + (double)R * drho) * inv * 0x1p-23;
+    G.fps_w = (int)std::ceil(span + 0.01) + 5;
+  }
   hipError_t e = hipMalloc(&G.d_lines, sizeof(Line) * (size_t)A);
   if (e == hipSuccess)
     e = hipMemcpy(G.d_lines, G.lines.data(), sizeof(Line) * (size_t)A, hipMemcpyHostToDevice);

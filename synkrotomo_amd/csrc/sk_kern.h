@@ -7,6 +7,8 @@
 // pixel index or candidate index depends on it.
 #pragma once
 
+#include <type_traits>
+
 #include "sk_common.h"
 
 namespace sk {
@@ -155,65 +157,110 @@ struct FpParams {
   const float* rowinv;  // FP_SIRT: [A][R]
 };
 
-// Fast-mode row walk of forwardprojection.fut:37-56 for RPT rays of one line, with the
-// line's slope sign as a template argument (Xmax = max(floor xbot, floor xtop) is then
-// floor xtop for slope >= 0, floor xbot otherwise).  Per row the ray contributes
-// fb*v(Xb) + (1 - fb)*v(Xt), fb = (Xmax - xbot)/(xtop - xbot) the fraction of the row's
-// length in the bottom pixel, accumulated as v(Xt) + fb*(v(Xb) - v(Xt)): a ray inside one
-// pixel (Xb == Xt, the reference's full-length case) adds that pixel once whatever fb is.
-// fb is clamped to [0, 1], which also maps the 0/0 of a zero-length range (slope 0) to 0.
-// The sums are fractions of the row length; the caller scales by lbase.
+// Lane k of every 16-lane row of v, broadcast to the whole wave when v holds the same 16
+// values in each row (DPP row_newbcast; the compiler folds it into the consuming add).
+template <int K>
+__device__ __forceinline__ float bcast16(float v) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + K, 0xf, 0xf, true));
+}
+
+// Fast-mode contribution of one row (forwardprojection.fut:37-56) given the row's xbot,
+// xtop and the two pixels the row can touch: vb = v(Xb), Xb = floor(xbot), and vo = its
+// neighbour on the slope's side, v(Xb + 1) for slope >= 0 and v(Xb - 1) otherwise.  The
+// row adds vb + fo*(vo - vb), fo = |xtop - Xmax| / |xtop - xbot| the fraction of its length
+// in the neighbour (the reference's Xmax = floor(xbot) + 1, resp. floor(xbot), when the
+// row crosses into it).  When the row stays in pixel Xb (the reference's full-length case)
+// the numerator has the wrong sign and the clamp to [0, 1] makes fo exactly 0, without a
+// select; a zero-length range (x/0 = inf, or 0*inf) also clamps to 0.  The sums are
+// fractions of the row length; the caller scales by lbase.
+template <bool POS>
+__device__ __forceinline__ float fp_row_add(float sum, float xb, float xt, float vb, float vo) {
+  const float fx = floorf(xb);
+  float fo;
+  if constexpr (POS) {  // (xtop - (fx + 1)) / (xtop - xbot) with one rounding: xtop - fx is exact
+    const float r = fast_rcp(xt - xb);
+    fo = fmaf(xt - fx, r, -r);
+  } else {              // (fx - xtop) / (xbot - xtop)
+    fo = (fx - xt) * fast_rcp(xb - xt);
+  }
+  fo = fminf(fmaxf(fo, 0.0f), 1.0f);
+  return fmaf(fo, vo - vb, sum + vb);
+}
+
+// Fast-mode row walk for RPT rays of one line, the slope sign a template argument.  Both
+// pixels of a row come from one column offset: the buffer loads at c and c + 1 (c = Xb
+// for slope >= 0, Xb - 1 otherwise), each range-checked by the row's descriptor.
+//
+// Rows go in blocks of 16.  The wave-uniform products i*slope of a block's rows are formed
+// once, one per lane of a 16-lane row, and each row's xbot = i*slope + icpt broadcasts its
+// lane into the add (one instruction per row instead of convert, multiply and add).  The
+// walk is software-pipelined three rows deep (four register sets): rows i+1..i+3 are in
+// flight while row i's fraction is formed.  Rows past i1 (the last block's tail and the
+// prefetches) read through an empty descriptor: no memory access, both values 0, and such
+// a row adds exactly 0.
 template <bool POS, int RPT>
 __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int i0, int i1,
                                              float slope, const float* icpt, float* sum) {
   if (i0 >= i1) return;
-  // Software-pipelined two rows deep, unrolled by three rows so no register copy waits on a
-  // load: rows i+1 and i+2 are in flight while row i's fractions are formed.  A prefetch
-  // past the last row uses an empty descriptor (no memory access, loads return 0).
-  float xb0[RPT], xt0[RPT], vb0[RPT], vt0[RPT];
-  float xb1[RPT], xt1[RPT], vb1[RPT], vt1[RPT];
-  float xb2[RPT], xt2[RPT], vb2[RPT], vt2[RPT];
-  auto gather = [&](int i, float* xbo, float* xto, float* vbo, float* vto) {
+  constexpr int D = 3;  // rows in flight
+  const int l16 = (int)(threadIdx.x & 15);
+  const int cofs = h - (POS ? 0 : 1);
+  float xb[D + 1][RPT], xt[D + 1][RPT], va[D + 1][RPT], vc[D + 1][RPT];
+  auto gather = [&](int i, float yslope, int q) {
     const __amdgpu_buffer_rsrc_t row = row_rsrc(src + (size_t)(i + h) * N, i < i1 ? N : 0);
-    const float yslope = (float)i * slope;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
-      xbo[k] = yslope + icpt[k];
-      xto[k] = xbo[k] + slope;
-      vbo[k] = row_load(row, floor_to_int(xbo[k]) + h);
-      vto[k] = row_load(row, floor_to_int(xto[k]) + h);
+      xb[q][k] = yslope + icpt[k];
+      xt[q][k] = xb[q][k] + slope;
+      const int c = (floor_to_int(xb[q][k]) + cofs) << 2;
+      // Two separate dword loads, each range-checked on its own offset (c may be -4): the
+      // empty asm hides the adjacency so they are not merged into one 8-byte access.
+      int c1 = c + 4;
+      asm volatile("" : "+v"(c1));
+      va[q][k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(row, c, 0, 0));
+      vc[q][k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(row, c1, 0, 0));
     }
   };
-  auto consume = [&](const float* xbi, const float* xti, const float* vbi, const float* vti) {
+  auto consume = [&](int q) {
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const float Xmax = floorf(POS ? xti[k] : xbi[k]);
-      const float fb = fminf(fmaxf((Xmax - xbi[k]) * fast_rcp(xti[k] - xbi[k]), 0.0f), 1.0f);
-      sum[k] = sum[k] + vti[k];
-      sum[k] = fmaf(fb, vbi[k] - vti[k], sum[k]);
-    }
+    for (int k = 0; k < RPT; ++k)
+      sum[k] = POS ? fp_row_add<true>(sum[k], xb[q][k], xt[q][k], va[q][k], vc[q][k])
+                   : fp_row_add<false>(sum[k], xb[q][k], xt[q][k], vc[q][k], va[q][k]);
   };
-  gather(i0, xb0, xt0, vb0, vt0);
-  gather(i0 + 1, xb1, xt1, vb1, vt1);
-  int i = i0;
-  // sched_barrier keeps the compiler from regrouping the stages (it would otherwise cluster
-  // the loads and wait on all of them at the top of the loop).
-  for (; i + 2 < i1; i += 3) {
-    gather(i + 2, xb2, xt2, vb2, vt2);
+  auto stage = [&](auto kc, int i, float tab) {
+    constexpr int K = decltype(kc)::value;
+    gather(i + D + K, bcast16<K>(tab), (K + D) % (D + 1));
     __builtin_amdgcn_sched_barrier(0);
-    consume(xb0, xt0, vb0, vt0);
+    consume(K % (D + 1));
     __builtin_amdgcn_sched_barrier(0);
-    gather(i + 3, xb0, xt0, vb0, vt0);
-    __builtin_amdgcn_sched_barrier(0);
-    consume(xb1, xt1, vb1, vt1);
-    __builtin_amdgcn_sched_barrier(0);
-    gather(i + 4, xb1, xt1, vb1, vt1);
-    __builtin_amdgcn_sched_barrier(0);
-    consume(xb2, xt2, vb2, vt2);
-    __builtin_amdgcn_sched_barrier(0);
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    gather(i0 + d, (float)(i0 + d) * slope, d);
+    __builtin_amdgcn_sched_barrier(0);  // issue in row order: the loop waits on the oldest
   }
-  if (i < i1) consume(xb0, xt0, vb0, vt0);
-  if (i + 1 < i1) consume(xb1, xt1, vb1, vt1);
+  for (int i = i0; i < i1; i += 16) {
+    // sched_barrier keeps the compiler from regrouping the stages (it would otherwise
+    // cluster the loads and wait on all of them at the top of the loop).
+    const float tab = (float)(i + D + l16) * slope;  // rows gathered in this block
+    stage(std::integral_constant<int, 0>{}, i, tab);
+    stage(std::integral_constant<int, 1>{}, i, tab);
+    stage(std::integral_constant<int, 2>{}, i, tab);
+    stage(std::integral_constant<int, 3>{}, i, tab);
+    stage(std::integral_constant<int, 4>{}, i, tab);
+    stage(std::integral_constant<int, 5>{}, i, tab);
+    stage(std::integral_constant<int, 6>{}, i, tab);
+    stage(std::integral_constant<int, 7>{}, i, tab);
+    stage(std::integral_constant<int, 8>{}, i, tab);
+    stage(std::integral_constant<int, 9>{}, i, tab);
+    stage(std::integral_constant<int, 10>{}, i, tab);
+    stage(std::integral_constant<int, 11>{}, i, tab);
+    stage(std::integral_constant<int, 12>{}, i, tab);
+    stage(std::integral_constant<int, 13>{}, i, tab);
+    stage(std::integral_constant<int, 14>{}, i, tab);
+    stage(std::integral_constant<int, 15>{}, i, tab);
+  }
 }
 
 // One wave's work item: line j, bins [wb*64*RPT, (wb+1)*64*RPT) of image bz (lane l: bins

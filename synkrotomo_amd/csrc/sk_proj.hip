@@ -58,6 +58,206 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
     fp_item<EXACT, MODE, WEIGHTS, DR1, RPT>(P, blockIdx.x, blockIdx.y * FP_WAVES + wave, blockIdx.z);
 }
 
+// ------------------------------------------------------------- staged A.x (fast mode)
+// A workgroup holds FPS_RAYS consecutive bins of one line, one ray per lane.  Frame rows go
+// in blocks of FPS_ROWS: for each row the columns any of the workgroup's rays can touch (a
+// segment of at most FPS_MAXW floats starting one column left of the lowest ray) are staged
+// in LDS with coalesced loads, zeros outside the image and past the last row, and every
+// lane reads its two pixels of the row from LDS.  The gathers thus leave the texture path,
+// which bounds the per-wave kernel.  The row arithmetic (fp_row_add, xbot from the
+// broadcast i*slope) is that of the per-wave kernel, and rows where a ray is outside the
+// image add exactly 0 in both, so the two give bitwise equal results.  NBUF = 2 double-
+// buffers the staged block (one barrier per block), NBUF = 1 halves the LDS per workgroup
+// (two barriers per block).
+constexpr int FPS_RAYS = 256;
+constexpr int FPS_ROWS = 16;
+constexpr int FPS_CPL = 6;                       // staged floats per lane and row
+constexpr int FPS_MAXW = FPS_CPL * 64;           // widest segment (floats): the LDS row stride
+
+typedef __attribute__((address_space(3))) float lds_float;
+
+template <int K>
+__device__ __forceinline__ int bcast16i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
+}
+
+template <bool POS, int NBUF>
+__device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
+                                          int i0w, int i1w, float slope, float icpt,
+                                          float icpt_lo, float* seg) {
+  const int tid = threadIdx.x;
+  const int l16 = tid & 15;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr int RPW = FPS_ROWS / (FPS_RAYS / 64);  // staged rows per wave
+  float st[RPW][FPS_CPL];
+  // Per block, lane l16 holds row ib + l16's i*slope (tab) and its segment start (sv, an
+  // image column one left of floor(x) of the lowest ray: xbot is rounded from i*slope + icpt
+  // exactly as each lane rounds it, and rounding is monotone, so every lane's
+  // floor(xbot) - 1 lies at or right of it).
+  auto block_rows = [&](int ib, float& tab, int& sv) {
+    tab = (float)(ib + l16) * slope;
+    sv = floor_to_int(tab + icpt_lo) + h - 1;
+  };
+  // Wave w stages rows RPW*w .. RPW*w + RPW-1 of the block, lane l columns s0 + l + 64m:
+  // coalesced dword loads through the row's descriptor (zero outside the image row and
+  // for rows past i1).
+  auto load = [&](int ib, int sv) {
+#pragma unroll
+    for (int t = 0; t < RPW; ++t) {
+      const int i = ib + RPW * wave + t;
+      const int s0 = __builtin_amdgcn_readlane(sv, RPW * wave + t);
+      const __amdgpu_buffer_rsrc_t rr = row_rsrc(src + (size_t)(i < i1 ? i + h : 0) * N, i < i1 ? N : 0);
+      const int vo = (s0 + lane) << 2;
+      if (s0 >= 0) {  // wave-uniform: non-negative offsets, the constant parts as immediates
+#pragma unroll
+        for (int m = 0; m < FPS_CPL; ++m)
+          st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, vo + 256 * m, 0, 0));
+      } else {
+        // segment starting left of the image: each load's full offset in its own register
+        // (a negative one is out of range as an unsigned offset and reads 0)
+#pragma unroll
+        for (int m = 0; m < FPS_CPL; ++m) {
+          int o = vo + 256 * m;
+          asm volatile("" : "+v"(o));
+          st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, o, 0, 0));
+        }
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < RPW; ++t) {
+      float* base = seg + (size_t)(buf * FPS_ROWS + RPW * wave + t) * FPS_MAXW + lane;
+#pragma unroll
+      for (int m = 0; m < FPS_CPL; ++m) base[64 * m] = st[t][m];
+    }
+  };
+  float acc = 0.0f;
+  const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
+  // A block's rows in two halves: the LDS reads of eight rows are issued together, and the
+  // second half's reads are in flight while the first half is summed.
+  float xbs[FPS_ROWS], vas[FPS_ROWS], vcs[FPS_ROWS];
+  auto fetch = [&](auto kc, float tab, int basel) {
+    constexpr int K = decltype(kc)::value;
+    xbs[K] = bcast16<K>(tab) + icpt;
+    // LDS byte address of the pair (basel includes the block's LDS base)
+    const unsigned off = (unsigned)(floor_to_int(xbs[K]) << 2) + (unsigned)bcast16i<K>(basel);
+    const lds_float* p = (const lds_float*)(uintptr_t)off;
+    vas[K] = p[0];
+    vcs[K] = p[1];
+  };
+  auto sum_row = [&](int k) {
+    const float xb = xbs[k], xt = xb + slope;
+    acc = POS ? fp_row_add<true>(acc, xb, xt, vas[k], vcs[k]) : fp_row_add<false>(acc, xb, xt, vcs[k], vas[k]);
+  };
+  auto compute = [&](int buf, float tab, int sv) {
+    // LDS byte address of the pair's first column for row ib + l16: 4*floor(xbot) + basel
+    const int basel = ((buf * FPS_ROWS + l16) * FPS_MAXW + h - sv - (POS ? 0 : 1)) * 4 + seg_lds;
+    fetch(std::integral_constant<int, 0>{}, tab, basel);
+    fetch(std::integral_constant<int, 1>{}, tab, basel);
+    fetch(std::integral_constant<int, 2>{}, tab, basel);
+    fetch(std::integral_constant<int, 3>{}, tab, basel);
+    fetch(std::integral_constant<int, 4>{}, tab, basel);
+    fetch(std::integral_constant<int, 5>{}, tab, basel);
+    fetch(std::integral_constant<int, 6>{}, tab, basel);
+    fetch(std::integral_constant<int, 7>{}, tab, basel);
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(std::integral_constant<int, 8>{}, tab, basel);
+    fetch(std::integral_constant<int, 9>{}, tab, basel);
+    fetch(std::integral_constant<int, 10>{}, tab, basel);
+    fetch(std::integral_constant<int, 11>{}, tab, basel);
+    fetch(std::integral_constant<int, 12>{}, tab, basel);
+    fetch(std::integral_constant<int, 13>{}, tab, basel);
+    fetch(std::integral_constant<int, 14>{}, tab, basel);
+    fetch(std::integral_constant<int, 15>{}, tab, basel);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum_row(k);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 8; k < 16; ++k) sum_row(k);
+  };
+  float tab, tabn;
+  int sv, svn;
+  block_rows(i0, tab, sv);
+  load(i0, sv);
+  store(0);
+  __syncthreads();
+  int b = 0;
+  for (int ib = i0; ib < i1; ib += FPS_ROWS, ++b) {  // i0, i1 uniform: every lane reaches each barrier
+    const bool more = ib + FPS_ROWS < i1;
+    if (more) {
+      block_rows(ib + FPS_ROWS, tabn, svn);
+      load(ib + FPS_ROWS, svn);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next block's loads stay ahead of the compute
+    // A wave skips blocks outside its own rays' row range: there they add exactly 0.
+    if (ib + FPS_ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sv);
+    if constexpr (NBUF == 1) __syncthreads();
+    if (more) store(NBUF == 2 ? ((b + 1) & 1) : 0);
+    __syncthreads();
+    tab = tabn;
+    sv = svn;
+  }
+  return acc;
+}
+
+// Grid: (lines, bin blocks of FPS_RAYS, images).  Dynamic LDS: NBUF * FPS_ROWS * FPS_MAXW floats.
+template <int MODE, int NBUF>
+__global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
+  extern __shared__ float4 seg4[];
+  const int j = blockIdx.x, bz = blockIdx.z;
+  const int first = blockIdx.y * FPS_RAYS;
+  const int r = first + (int)threadIdx.x;
+  const Line L = P.lines[j];
+  const int N = P.N, h = N >> 1;
+  const float cost = L.cost, slope = L.slope;
+  const float icpt = (P.rho0 + (float)r * P.drho) / cost;  // forwardprojection.fut:33-34
+  // The workgroup's intercepts are monotone in r: the extremes are lanes 0 and FPS_RAYS-1.
+  const float ia = (P.rho0 + (float)first * P.drho) / cost;
+  const float iz = (P.rho0 + (float)(first + FPS_RAYS - 1) * P.drho) / cost;
+  const float icpt_lo = fminf(ia, iz);
+  int i0, i1, i0w, i1w;
+  {
+    // the workgroup's row range (hull over its rays) and the wave's own
+    int a0, a1, b0, b1;
+    const int last = min(first + FPS_RAYS - 1, P.R - 1);
+    row_clip(slope, ia, h, a0, a1);
+    row_clip(slope, (P.rho0 + (float)last * P.drho) / cost, h, b0, b1);
+    i0 = __builtin_amdgcn_readfirstlane(min(a0, b0));
+    i1 = __builtin_amdgcn_readfirstlane(max(a1, b1));
+    const int wfirst = first + (int)(threadIdx.x & ~63u);
+    const int wlast = min(wfirst + 63, P.R - 1);
+    if (wfirst > wlast) {
+      i0w = i1w = i0;  // no valid ray in this wave
+    } else {
+      row_clip(slope, (P.rho0 + (float)wfirst * P.drho) / cost, h, a0, a1);
+      row_clip(slope, (P.rho0 + (float)wlast * P.drho) / cost, h, b0, b1);
+      i0w = min(a0, b0);
+      i1w = max(a1, b1);
+    }
+    i0w = __builtin_amdgcn_readfirstlane(i0w);
+    i1w = __builtin_amdgcn_readfirstlane(i1w);
+  }
+  const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+  float* seg = (float*)seg4;
+  float acc = 0.0f;
+  if (i0 < i1) {
+    if (L.flags & LINE_SLOPE_POS)
+      acc = fps_walk<true, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+    else
+      acc = fps_walk<false, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+  }
+  if (r >= P.R) return;
+  const float s = acc * L.lbase;
+  const size_t o = (size_t)bz * P.A * P.R + (size_t)L.orig * P.R + r;
+  if constexpr (MODE == FP_PLAIN)
+    P.out[o] = s;
+  else
+    P.out[o] = P.rowinv[(size_t)L.orig * P.R + r] * (P.b[o] - s);  // SIRT.fut:33-36
+}
+
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256) void k_bp(BpParams P) {
@@ -116,9 +316,27 @@ static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) 
     fp_go<EXACT, MODE, W_REF, true, 1>(g, a, s);
 }
 
+template <int MODE, int NBUF>
+static void fp_go_staged(const Geom* g, const FpArgs& a, hipStream_t s) {
+  const unsigned nb = (unsigned)((g->R + FPS_RAYS - 1) / FPS_RAYS);
+  const size_t lds = sizeof(float) * NBUF * FPS_ROWS * (size_t)FPS_MAXW;
+  hipLaunchKernelGGL((k_fp_staged<MODE, NBUF>), dim3(g->A, nb, a.batch), dim3(FPS_RAYS), lds, s,
+                     fp_params(g, a));
+}
+
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s) {
   const bool ex = c->exact != 0;
-  const int v = g->fp_variant >= 0 ? g->fp_variant : c->fp_variant;
+  int v = g->fp_variant >= 0 ? g->fp_variant : c->fp_variant;
+  if (v == 5 || v == 6) {
+    if (!ex && g->fps_w > 0 && g->fps_w <= FPS_MAXW && mode != FP_ONES_INV) {
+      if (mode == FP_PLAIN)
+        v == 5 ? fp_go_staged<FP_PLAIN, 2>(g, a, s) : fp_go_staged<FP_PLAIN, 1>(g, a, s);
+      else
+        v == 5 ? fp_go_staged<FP_SIRT, 2>(g, a, s) : fp_go_staged<FP_SIRT, 1>(g, a, s);
+      return launched(c, "forward projection kernel");
+    }
+    v = 3;  // exact mode, or a geometry whose segments do not fit: the per-wave kernel
+  }
   switch (mode) {
     case FP_PLAIN: ex ? fp_go_v<true, FP_PLAIN>(v, g, a, s) : fp_go_v<false, FP_PLAIN>(v, g, a, s); break;
     case FP_ONES_INV:
