@@ -72,7 +72,8 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp(FpParams P) {
 constexpr int FPS_RAYS = 256;
 constexpr int FPS_ROWS = 16;
 constexpr int FPS_CPL = 6;                       // staged floats per lane and row
-constexpr int FPS_MAXW = FPS_CPL * 64;           // widest segment (floats): the LDS row stride
+constexpr int FPS_MAXW = FPS_CPL * 64;           // widest segment (floats)
+constexpr int FPS_CS = FPS_ROWS + 1;             // LDS stride of a staged column (floats)
 
 typedef __attribute__((address_space(3))) float lds_float;
 
@@ -91,22 +92,23 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int RPW = FPS_ROWS / (FPS_RAYS / 64);  // staged rows per wave
   float st[RPW][FPS_CPL];
-  // Per block, lane l16 holds row ib + l16's i*slope (tab) and its segment start (sv, an
-  // image column one left of floor(x) of the lowest ray: xbot is rounded from i*slope + icpt
-  // exactly as each lane rounds it, and rounding is monotone, so every lane's
-  // floor(xbot) - 1 lies at or right of it).
-  auto block_rows = [&](int ib, float& tab, int& sv) {
+  // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segments all start at
+  // image column sb, one left of floor(x) of the lowest ray over the block's rows (xbot is
+  // rounded from i*slope + icpt exactly as each lane rounds it, and rounding is monotone, so
+  // every lane's floor(xbot) - 1 lies at or right of it).
+  auto block_rows = [&](int ib, float& tab, int& sb) {
     tab = (float)(ib + l16) * slope;
-    sv = floor_to_int(tab + icpt_lo) + h - 1;
+    const int sv = floor_to_int(tab + icpt_lo) + h - 1;
+    // one start for the whole block: the lower of its first and last rows' (monotone in i)
+    sb = min(__builtin_amdgcn_readlane(sv, 0), __builtin_amdgcn_readlane(sv, FPS_ROWS - 1));
   };
   // Wave w stages rows RPW*w .. RPW*w + RPW-1 of the block, lane l columns s0 + l + 64m:
   // coalesced dword loads through the row's descriptor (zero outside the image row and
   // for rows past i1).
-  auto load = [&](int ib, int sv) {
+  auto load = [&](int ib, int s0) {
 #pragma unroll
     for (int t = 0; t < RPW; ++t) {
       const int i = ib + RPW * wave + t;
-      const int s0 = __builtin_amdgcn_readlane(sv, RPW * wave + t);
       const __amdgpu_buffer_rsrc_t rr = row_rsrc(src + (size_t)(i < i1 ? i + h : 0) * N, i < i1 ? N : 0);
       const int vo = (s0 + lane) << 2;
       if (s0 >= 0) {  // wave-uniform: non-negative offsets, the constant parts as immediates
@@ -125,85 +127,90 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
       }
     }
   };
+  // LDS layout of a staged block: column-interleaved, column j of row slot r at j*CS + r
+  // (CS = FPS_ROWS + 1 spreads consecutive columns over the banks), so a lane's pair in row K
+  // is one address plus the constant offsets K and K + CS.
   auto store = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < RPW; ++t) {
-      float* base = seg + (size_t)(buf * FPS_ROWS + RPW * wave + t) * FPS_MAXW + lane;
+      float* base = seg + (size_t)buf * FPS_MAXW * FPS_CS + lane * FPS_CS + RPW * wave + t;
 #pragma unroll
-      for (int m = 0; m < FPS_CPL; ++m) base[64 * m] = st[t][m];
+      for (int m = 0; m < FPS_CPL; ++m) base[64 * FPS_CS * m] = st[t][m];
     }
   };
   float acc = 0.0f;
   const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
-  // A block's rows in two halves: the LDS reads of eight rows are issued together, and the
-  // second half's reads are in flight while the first half is summed.
-  float xbs[FPS_ROWS], vas[FPS_ROWS], vcs[FPS_ROWS];
-  auto fetch = [&](auto kc, float tab, int basel) {
+  // The LDS reads run eight rows ahead of the sums (rolling window of eight row registers).
+  float xbs[8], vas[8], vcs[8];
+  auto fetch = [&](auto kc, float tab, int bbase) {
     constexpr int K = decltype(kc)::value;
-    xbs[K] = bcast16<K>(tab) + icpt;
-    // LDS byte address of the pair (basel includes the block's LDS base)
-    const unsigned off = (unsigned)(floor_to_int(xbs[K]) << 2) + (unsigned)bcast16i<K>(basel);
+    constexpr int S = K & 7;
+    xbs[S] = bcast16<K>(tab) + icpt;
+    // LDS byte address of the pair's first column in row slot 0
+    const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * FPS_CS) + bbase);
     const lds_float* p = (const lds_float*)(uintptr_t)off;
-    vas[K] = p[0];
-    vcs[K] = p[1];
+    vas[S] = p[K];
+    vcs[S] = p[K + FPS_CS];
   };
-  auto sum_row = [&](int k) {
-    const float xb = xbs[k], xt = xb + slope;
-    acc = POS ? fp_row_add<true>(acc, xb, xt, vas[k], vcs[k]) : fp_row_add<false>(acc, xb, xt, vcs[k], vas[k]);
+  auto sum_row = [&](int S) {
+    const float xb = xbs[S], xt = xb + slope;
+    acc = POS ? fp_row_add<true>(acc, xb, xt, vas[S], vcs[S]) : fp_row_add<false>(acc, xb, xt, vcs[S], vas[S]);
   };
-  auto compute = [&](int buf, float tab, int sv) {
-    // LDS byte address of the pair's first column for row ib + l16: 4*floor(xbot) + basel
-    const int basel = ((buf * FPS_ROWS + l16) * FPS_MAXW + h - sv - (POS ? 0 : 1)) * 4 + seg_lds;
-    fetch(std::integral_constant<int, 0>{}, tab, basel);
-    fetch(std::integral_constant<int, 1>{}, tab, basel);
-    fetch(std::integral_constant<int, 2>{}, tab, basel);
-    fetch(std::integral_constant<int, 3>{}, tab, basel);
-    fetch(std::integral_constant<int, 4>{}, tab, basel);
-    fetch(std::integral_constant<int, 5>{}, tab, basel);
-    fetch(std::integral_constant<int, 6>{}, tab, basel);
-    fetch(std::integral_constant<int, 7>{}, tab, basel);
+  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + 8
+    constexpr int K = decltype(kc)::value;
+    sum_row(K);
+    fetch(std::integral_constant<int, K + 8>{}, tab, bbase);
     __builtin_amdgcn_sched_barrier(0);
-    fetch(std::integral_constant<int, 8>{}, tab, basel);
-    fetch(std::integral_constant<int, 9>{}, tab, basel);
-    fetch(std::integral_constant<int, 10>{}, tab, basel);
-    fetch(std::integral_constant<int, 11>{}, tab, basel);
-    fetch(std::integral_constant<int, 12>{}, tab, basel);
-    fetch(std::integral_constant<int, 13>{}, tab, basel);
-    fetch(std::integral_constant<int, 14>{}, tab, basel);
-    fetch(std::integral_constant<int, 15>{}, tab, basel);
+  };
+  auto compute = [&](int buf, float tab, int sb) {
+    // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
+    const int bbase = (buf * FPS_MAXW * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
+    fetch(std::integral_constant<int, 0>{}, tab, bbase);
+    fetch(std::integral_constant<int, 1>{}, tab, bbase);
+    fetch(std::integral_constant<int, 2>{}, tab, bbase);
+    fetch(std::integral_constant<int, 3>{}, tab, bbase);
+    fetch(std::integral_constant<int, 4>{}, tab, bbase);
+    fetch(std::integral_constant<int, 5>{}, tab, bbase);
+    fetch(std::integral_constant<int, 6>{}, tab, bbase);
+    fetch(std::integral_constant<int, 7>{}, tab, bbase);
     __builtin_amdgcn_sched_barrier(0);
+    step(std::integral_constant<int, 0>{}, tab, bbase);
+    step(std::integral_constant<int, 1>{}, tab, bbase);
+    step(std::integral_constant<int, 2>{}, tab, bbase);
+    step(std::integral_constant<int, 3>{}, tab, bbase);
+    step(std::integral_constant<int, 4>{}, tab, bbase);
+    step(std::integral_constant<int, 5>{}, tab, bbase);
+    step(std::integral_constant<int, 6>{}, tab, bbase);
+    step(std::integral_constant<int, 7>{}, tab, bbase);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) sum_row(k);
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 8; k < 16; ++k) sum_row(k);
+    for (int S = 0; S < 8; ++S) sum_row(S);
   };
   float tab, tabn;
-  int sv, svn;
-  block_rows(i0, tab, sv);
-  load(i0, sv);
+  int sb, sbn;
+  block_rows(i0, tab, sb);
+  load(i0, sb);
   store(0);
   __syncthreads();
   int b = 0;
   for (int ib = i0; ib < i1; ib += FPS_ROWS, ++b) {  // i0, i1 uniform: every lane reaches each barrier
     const bool more = ib + FPS_ROWS < i1;
     if (more) {
-      block_rows(ib + FPS_ROWS, tabn, svn);
-      load(ib + FPS_ROWS, svn);
+      block_rows(ib + FPS_ROWS, tabn, sbn);
+      load(ib + FPS_ROWS, sbn);
     }
     __builtin_amdgcn_sched_barrier(0);  // the next block's loads stay ahead of the compute
     // A wave skips blocks outside its own rays' row range: there they add exactly 0.
-    if (ib + FPS_ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sv);
+    if (ib + FPS_ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sb);
     if constexpr (NBUF == 1) __syncthreads();
     if (more) store(NBUF == 2 ? ((b + 1) & 1) : 0);
     __syncthreads();
     tab = tabn;
-    sv = svn;
+    sb = sbn;
   }
   return acc;
 }
 
-// Grid: (lines, bin blocks of FPS_RAYS, images).  Dynamic LDS: NBUF * FPS_ROWS * FPS_MAXW floats.
+// Grid: (lines, bin blocks of FPS_RAYS, images).  Dynamic LDS: NBUF * FPS_CS * FPS_MAXW floats.
 template <int MODE, int NBUF>
 __global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
   extern __shared__ float4 seg4[];
@@ -319,7 +326,7 @@ static void fp_go_v(int variant, const Geom* g, const FpArgs& a, hipStream_t s) 
 template <int MODE, int NBUF>
 static void fp_go_staged(const Geom* g, const FpArgs& a, hipStream_t s) {
   const unsigned nb = (unsigned)((g->R + FPS_RAYS - 1) / FPS_RAYS);
-  const size_t lds = sizeof(float) * NBUF * FPS_ROWS * (size_t)FPS_MAXW;
+  const size_t lds = sizeof(float) * NBUF * FPS_CS * (size_t)FPS_MAXW;
   hipLaunchKernelGGL((k_fp_staged<MODE, NBUF>), dim3(g->A, nb, a.batch), dim3(FPS_RAYS), lds, s,
                      fp_params(g, a));
 }
