@@ -88,6 +88,29 @@ def sparse_leg(geo, x, sino, K: int) -> dict:
     return res
 
 
+def sirt_leg(geo, x_true, sino, iters: int = 50) -> dict:
+    """Plain SIRT at the same geometry: ``iters`` iterations from a zero image on the
+    phantom's sinogram (SIRT.main), timed with HIP events on the launch stream after one
+    untimed call (weights and graph capture).  Reports the data residual ||Ax - b|| / ||b||
+    and the image error against the phantom, both in FP64."""
+    import torch
+    img = torch.zeros((geo.N, geo.N), device="cuda")
+    geo.sirt_(img, sino, iters)  # SIRT weights, graph capture, warm caches
+    img.zero_()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    geo.sirt_(img, sino, iters)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    res = float((geo.fp(img) - sino).double().norm() / sino.double().norm())
+    err = float((img - x_true).double().norm() / x_true.double().norm())
+    return {"iterations": iters, "ms": round(ms, 3), "ms_per_iter": round(ms / iters, 3),
+            "gups": round(2.0 * float(N) * N * A * iters / (ms * 1e-3) / 1e9, 2),
+            "residual": round(res, 6), "image_rel_err": round(err, 6)}
+
+
 def _batched_case(ctx, specs, iters: int, seed: int) -> dict:
     """specs: [(N, angles, images)], centred unit bins.  Batched SIRT (one persistent launch
     sequence, one hipGraph) against the same problems group by group through sk_sirt with
@@ -207,6 +230,7 @@ def main() -> int:
     ap.add_argument("--no-sparse", action="store_true", help="skip the explicit-matrix leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-SIRT leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-SIRT leg")
+    ap.add_argument("--no-sirt", action="store_true", help="skip the 2048^2 SIRT leg")
     ap.add_argument("--batched-iters", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use default kernel variants")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -294,10 +318,13 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes()},
         "cpu_baseline": None,
+        "sirt": None,
         "sparse": None,
         "batched": None,
         "learned": None,
     }
+    if not args.no_sirt:
+        out["sirt"] = sirt_leg(geo, x, sino)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx)
     if not args.no_batched:

@@ -10,6 +10,11 @@
 //               (skipped rows contribute exactly +0 in the reference).  Flat lines read a
 //               transposed copy of the image, the reference's own frame trick
 //               (forwardprojection.fut:80-82).
+//   k_fp_staged A.x, staged (variants 5, 6): a workgroup = 256 consecutive bins of one
+//               line.  It walks the frame rows in blocks of 16: the four waves load the
+//               block's image-row segment (the columns any of its rays can touch) with
+//               coalesced dword loads into LDS, then every lane forms its ray's row steps
+//               from LDS.  Same per-step expressions as k_fp (bitwise equal results).
 //   k_bp  B.r   (futhark/backprojection.fut:20-93).  Pixel-parallel gather, no atomics: a
 //               block owns a 16 x 16*PPT pixel tile and walks the lines in order, staging
 //               for each block of 32 lines the tile's rho-window of the sinogram in LDS

@@ -67,6 +67,8 @@ def train(args) -> int:
     scale = attenuation_scale(ctx, geo, np.random.default_rng(TRAIN_SEED + 1))
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED)
     net = Correction(ctx, seed=0)
+    if args.init:
+        net.load(args.init)  # fine-tune from earlier weights
     model = LearnedSIRT(geo, net, args.stages)
     steps = args.epochs * (args.phantoms // args.batch)
     opt = torch.optim.Adam(net.parameters(), lr=args.lr)
@@ -77,10 +79,18 @@ def train(args) -> int:
         if step % (args.phantoms // args.batch) == 0:
             rng = np.random.default_rng(TRAIN_SEED)  # each epoch revisits the same phantoms
         x_true = gpu_phantoms(ctx, rng, args.batch, N, ELLIPSES)
-        b = noisy_sinogram(geo.fp(x_true), scale, I0, gen)
+        clean = geo.fp(x_true)
+        b = noisy_sinogram(clean, scale, I0, gen)
         stages = model.unrolled(b, checkpoint=True, all_stages=True)
         x = stages[-1]
         loss = torch.mean((x - x_true) ** 2)
+        if args.proj_weight > 0:
+            # projection-domain error against the noise-free data: ||A(x - x_true)||^2 /
+            # ||A x_true||^2 per image (the part of the data residual that is not noise)
+            from .autograd import project
+            cc = clean.reshape(b.shape[0], -1)
+            d = project(x, geo).reshape(b.shape[0], -1) - cc
+            loss = loss + args.proj_weight * torch.mean(d.square().sum(1) / cc.square().sum(1))
         if args.stage_weight > 0:
             for xs in stages[:-1]:
                 loss = loss + args.stage_weight * torch.mean((xs - x_true) ** 2)
@@ -113,7 +123,8 @@ def train(args) -> int:
     net.save(args.out, scale=scale, I0=I0, stages=args.stages, N=N, A=A, phantoms=args.phantoms,
              batch=args.batch, epochs=args.epochs, res_weight=args.res_weight,
              stage_weight=args.stage_weight, res_excess_weight=args.res_excess_weight,
-             train_seconds=train_s)
+             proj_weight=args.proj_weight, init=os.path.basename(args.init) if args.init else "",
+             lr=args.lr, train_seconds=train_s)
     print(json.dumps({"saved": args.out, "train_seconds": round(train_s, 1), "scale": scale}), flush=True)
     return 0
 
@@ -251,6 +262,10 @@ def main(argv=None) -> int:
     t.add_argument("--res-excess-weight", type=float, default=0.0,
                    help="weight of the squared excess of the final residual over 1.1x the "
                         "phantom's noise floor")
+    t.add_argument("--proj-weight", type=float, default=0.0,
+                   help="weight of the final image's projection error against the noise-free "
+                        "data, ||A(x - x_true)||^2 / ||A x_true||^2")
+    t.add_argument("--init", default="", help="start from these weights instead of a fresh init")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     e = sub.add_parser("eval")
     e.add_argument("--phantoms", type=int, default=1000)

@@ -98,3 +98,31 @@ def test_sirt_batch_equals_single(ctx):
         x = torch.zeros(N, N, device="cuda")
         geo.sirt_(x, b[i].contiguous(), 6)
         torch.testing.assert_close(xb[i], x, rtol=0, atol=0)
+
+
+def test_sirt_512_variant_vs_oracles(ctx):
+    """The 512^2 variant of the 2048^2 SIRT configuration: 50-ellipse phantom, 512 angles in
+    [0, pi), 725 bins, rhozero -362, sinogram from the FP32 oracle's forward projector, 20
+    iterations from a zero image (the oracle needs about 1 s per iteration on 16 cores; the
+    2048^2 run itself is timed by bench.py).  GPU (fast mode) vs the FP32 oracle within 1e-5;
+    the FP64 oracle's distance to both is printed."""
+    from synkrotomo_amd.phantom import ellipse_phantom
+    from synkrotomo_amd.projector import uniform_angles
+    N, A, R, rz, iters = 512, 512, 725, -362.0, 20
+    ang = uniform_angles(A)
+    ph = ellipse_phantom(N, 50, seed=512).ravel()
+    nt = min(16, oracle.default_threads())
+    b = oracle.fp(ang, rz, 1.0, R, ph, nthreads=nt)
+    geo = ctx.geometry(ang, rz, 1.0, R, N)
+    np.testing.assert_allclose(geo.fp(_dev(ph)).cpu().numpy().ravel(), b, rtol=0,
+                               atol=1e-5 * float(np.abs(b).max()))
+    x = torch.zeros(N * N, device="cuda")
+    geo.sirt_(x, _dev(b), iters)
+    got = x.cpu().numpy()
+    ref32 = oracle.sirt(ang, rz, 1.0, np.zeros(N * N, np.float32), b, iters, nthreads=nt)
+    ref64 = oracle.sirt_f64(ang, rz, 1.0, np.zeros(N * N), b.astype(np.float64), iters, nthreads=nt)
+    err = rel_l2(got, ref32)
+    print(f"SIRT 512^2/512/725, {iters} it: fast vs FP32 oracle {err:.3e}; vs FP64 oracle "
+          f"{rel_l2(got, ref64):.3e} (FP32 oracle vs FP64 {rel_l2(ref32, ref64):.3e}); "
+          f"image error vs phantom {rel_l2(got, ph):.3e}")
+    assert err <= TOL
