@@ -16,7 +16,11 @@ namespace sk {
 enum : int {
   LINE_FLAT = 1,      // flat angle: the line acts on the transposed image
   LINE_SLOPE_POS = 2, // slope >= 0
-  LINE_DEGEN = 4      // |slope| < 1e-3: x_top may round onto x_bot (theta ~ 0, pi/2)
+  LINE_DEGEN = 4,     // |slope| < 1e-3: x_top may round onto x_bot (theta ~ 0, pi/2)
+  LINE_FLAG_BITS = 7,
+  // bits >= LINE_RUN_SHIFT: number of lines from this one to the end of its run of
+  // consecutive table entries with equal flag bits (>= 1)
+  LINE_RUN_SHIFT = 8
 };
 
 // One line in its frame (futhark/sirtlib.fut:39-41): steep angles keep the image frame

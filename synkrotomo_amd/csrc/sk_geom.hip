@@ -78,6 +78,12 @@ static int build_lines(const float* ang, int A, std::vector<Line>& out) {
   }
   out = steep;
   out.insert(out.end(), flat.begin(), flat.end());
+  // run lengths (consecutive lines with equal flag bits), counted from the back
+  for (int j = A - 1; j >= 0; j--) {
+    const bool same = j + 1 < A && (out[j + 1].flags & LINE_FLAG_BITS) == (out[j].flags & LINE_FLAG_BITS);
+    const int run = same ? (out[j + 1].flags >> LINE_RUN_SHIFT) + 1 : 1;
+    out[j].flags = (out[j].flags & LINE_FLAG_BITS) | (run << LINE_RUN_SHIFT);
+  }
   return (int)steep.size();
 }
 
@@ -147,7 +153,7 @@ int sk_lines_compute(const float* angles_host, int A, float* table_out) {
     const Line& L = lines[j];
     float* t = table_out + 8 * j;
     t[0] = L.cost; t[1] = L.sint; t[2] = L.lbase; t[3] = L.slope;
-    t[4] = L.fc; t[5] = L.fs; t[6] = (float)L.orig; t[7] = (float)L.flags;
+    t[4] = L.fc; t[5] = L.fs; t[6] = (float)L.orig; t[7] = (float)(L.flags & LINE_FLAG_BITS);
   }
   return ns;
 }

@@ -489,72 +489,79 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         x1[q] = xm[q] + 1.0f;
         acc[q] = part ? acc_f[q] : acc_s[q];
       }
-      for (int jj = j_begin; jj < j_end; ++jj) {
+      if constexpr (REV) {
+        // The table keeps the lines of each part in angle order, so the lines of one slope
+        // sign (degenerate or not) form runs; Line::flags >> LINE_RUN_SHIFT is the number of
+        // lines from a line to the end of its run.  Each run is walked by a loop whose body
+        // has the sign and the degenerate case resolved at compile time (no per-line branch).
+        auto run = [&](auto kind, int jb, int je) {
+          constexpr int KIND = decltype(kind)::value;  // 0: slope >= 0, 1: slope < 0, 2: degenerate
+          for (int jj = jb; jj < je; ++jj) {
+            const float slope = P.lines[l0 + jj].slope;
+            const BpLineLds lc = lsh[jj];
+            // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
+            const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
+            // All of the lane's pixels first compute their candidate index and read their
+            // candidates: the LDS reads of all pixels are in flight together.  Bin s sits at
+            // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of
+            // the pixel whose first candidate is s (w0 is uniform: scalar base arithmetic).
+            const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
+            float2 ev[PPT][NC];
+            float xbase[PPT];
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+              // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this
+              // path), so its ceiling lies in [w0, w0 + W - NC]: no clamp.
+              const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
+              const float2* wrow = wrev + neg_ceil_to_int(t);
+              xbase[q] = ym[q] * slope;  // backprojection.fut:55
+#pragma unroll
+              for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
+            }
+            // Non-degenerate lines: max(min(a, b), 0) == med3(a, b, 0): a and b cannot both
+            // be negative when the ray's x-range has positive length; m/diff is then >= 0,
+            // so min(., 1) is a clamp to [0, 1].
+#pragma unroll
+            for (int q = 0; q < PPT; ++q)
+#pragma unroll
+              for (int k = 0; k < NC; ++k) {
+                const float xb = xbase[q] + ev[q][k].x;
+                const float xt = xb + slope;
+                if constexpr (KIND == 0) {
+                  const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
+                  const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
+                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+                } else if constexpr (KIND == 1) {
+                  const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                  const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
+                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+                } else {
+                  // degenerate line: general form with the m == 0 case, lbase already in e.y
+                  const float l = cand_len<false, W_REF>(xb, slope, 1.0f, xm[q], x1[q], true);
+                  acc[q] = fmaf(l, ev[q][k].y, acc[q]);
+                }
+              }
+          }
+        };
+        for (int jr = j_begin; jr < j_end;) {  // uniform: scalar loop over the runs
+          const int f = P.lines[l0 + jr].flags;
+          const int je = min(jr + (f >> LINE_RUN_SHIFT), j_end);
+          if (f & LINE_DEGEN)
+            run(std::integral_constant<int, 2>{}, jr, je);
+          else if (f & LINE_SLOPE_POS)
+            run(std::integral_constant<int, 0>{}, jr, je);
+          else
+            run(std::integral_constant<int, 1>{}, jr, je);
+          jr = je;
+        }
+      }
+      if constexpr (!REV) for (int jj = j_begin; jj < j_end; ++jj) {
         const Line L = P.lines[l0 + jj];
         const BpLineLds lc = lsh[jj];
         const bool degen = L.flags & LINE_DEGEN;
         const bool pos = L.flags & LINE_SLOPE_POS;
         // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
         const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
-        if constexpr (REV) {
-          // All of the lane's pixels first compute their candidate index and read their
-          // candidates, then one branch per line (degenerate / slope sign) covers every
-          // pixel: the LDS reads of all pixels are in flight together.  Bin s sits at
-          // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of
-          // the pixel whose first candidate is s (w0 is uniform: scalar base arithmetic).
-          const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
-          float2 ev[PPT][NC];
-          float xbase[PPT];
-#pragma unroll
-          for (int q = 0; q < PPT; ++q) {
-            // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this path),
-            // so its ceiling lies in [w0, w0 + W - NC]: no clamp.
-            const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
-            const float2* wrow = wrev + neg_ceil_to_int(t);
-            xbase[q] = ym[q] * L.slope;  // backprojection.fut:55
-#pragma unroll
-            for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
-          }
-          if (!degen) {
-            // max(min(a, b), 0) == med3(a, b, 0): a and b cannot both be negative when the
-            // ray's x-range has positive length; m/diff is then >= 0, so min(., 1) is a
-            // clamp to [0, 1].
-            if (pos) {
-#pragma unroll
-              for (int q = 0; q < PPT; ++q)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) {
-                  const float xb = xbase[q] + ev[q][k].x;
-                  const float xt = xb + L.slope;
-                  const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
-                  const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
-                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-                }
-            } else {
-#pragma unroll
-              for (int q = 0; q < PPT; ++q)
-#pragma unroll
-                for (int k = 0; k < NC; ++k) {
-                  const float xb = xbase[q] + ev[q][k].x;
-                  const float xt = xb + L.slope;
-                  const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
-                  const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
-                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-                }
-            }
-          } else {
-            // degenerate line: general form with the m == 0 case, lbase already in e.y
-#pragma unroll
-            for (int q = 0; q < PPT; ++q)
-#pragma unroll
-              for (int k = 0; k < NC; ++k) {
-                const float l = cand_len<false, W_REF>(xbase[q] + ev[q][k].x, L.slope, 1.0f, xm[q],
-                                                       x1[q], true);
-                acc[q] = fmaf(l, ev[q][k].y, acc[q]);
-              }
-          }
-          continue;
-        }
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
           const float xmin = xm[q], ymin = ym[q], xmin1 = x1[q];
