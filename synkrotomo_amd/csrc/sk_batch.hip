@@ -4,10 +4,12 @@
 // Batched SIRT (no reference counterpart; generalizes SIRT3D.fut:18-28, which maps SIRT over
 // the slices of one geometry): a problem set holds groups of independent problems, each
 // group with its own geometry (image size, angles, bins).  One iteration is two persistent
-// launches sized to fill the chip once: in k_fp_persist every wave walks a static stride of
-// (problem, image, line, 64-bin group) items; in k_bp_persist every block pulls (problem,
-// image, tile) items from a global work counter, queued largest-problem first so the tail
-// of the queue is made of the shortest items.  The fp kernel resets the bp counter and the
+// launches sized to fill the chip once: in k_fps_persist (fast mode) every workgroup pulls
+// (problem, image, line, 256-bin block) items of the staged forward projector from a global
+// work counter (k_fp_persist: waves pull 64-bin items of the per-wave walk, exact mode or
+// segments wider than the staging buffer); in k_bp_persist every block pulls (problem,
+// image, tile) items, queued largest-problem first so the tail of the queue is made of the
+// shortest items.  The fp kernel resets the bp counter and the
 // bp kernel the fp slot, so the whole iteration sequence (a reset kernel, then fp/bp pairs)
 // is captured once into a hipGraph and launched once.
 //
@@ -101,6 +103,31 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchArgs a, unsi
   }
 }
 
+// Staged forward items (fast mode, every group's segments fit FPS_MAXW): one workgroup per
+// item of FPS_RAYS bins of one line of one image, claimed one at a time from the global
+// counter (the claim is broadcast through LDS; its two barriers are workgroup-uniform), in
+// the same lines-fastest order.  NBUF = 1: FPS_CS * FPS_MAXW floats of dynamic LDS.
+__global__ __launch_bounds__(FPS_RAYS) void k_fps_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
+  extern __shared__ float4 seg4[];
+  __shared__ unsigned item_sh;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0u;  // the bp queue of this iteration
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[2], 1u);
+    __syncthreads();
+    const long item = item_sh;
+    __syncthreads();
+    if (item >= a.fp_total) break;
+    const BatchProb& B = a.p[find_prob<true>(a, item)];
+    const long local = item - B.fp_begin;
+    const long per_img = (long)B.fp.A * B.fp_wb;
+    const int bz = (int)(local / per_img);
+    const long rem = local - (long)bz * per_img;
+    const int wb = (int)(rem / B.fp.A);
+    const int j = (int)(rem - (long)wb * B.fp.A);
+    fps_item<FP_SIRT, 1>(B.fp, u32(j), u32(wb), u32(bz), (float*)seg4);
+  }
+}
+
 template <bool EXACT, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
@@ -126,20 +153,28 @@ __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned*
 
 struct BatchPlan {
   BatchArgs args{};
-  // bit0: 2 rays per lane, bit1: dynamic claiming.  2 measured fastest on the batched
-  // configuration (3.82 s vs 4.02 s for mode 3 and 5.10 s for mode 0: static striding leaves
-  // waves with the costliest items idle-waiting at the end); SYNKRO_BATCH_FP_MODE overrides it.
+  // Per-wave forward kernel: bit0 2 rays per lane, bit1 dynamic claiming.  2 measured fastest
+  // of those on the batched configuration (3.82 s vs 4.02 s for mode 3 and 5.10 s for mode 0:
+  // static striding leaves waves with the costliest items idle-waiting at the end).
+  // SYNKRO_BATCH_FP_MODE overrides it; 4 = the staged kernel, the default in fast mode when
+  // every group's segments fit.
   int fp_mode = 2;
+  int bp_ppt = 4;
   unsigned* d_ctr = nullptr;
   int Wmax = 0;
   bool dr1 = true;
 };
 
-constexpr int BATCH_PPT = 2;
+// Pixels per lane of the persistent backprojector (16 x 16*PPT tiles): 4 by default (16x64
+// tiles: 3.11 s vs 3.23 s with 16x32 on the four large groups of the batched configuration,
+// equal on small groups); SYNKRO_BATCH_BP_PPT=2 selects 16x32 tiles.
 
 template <bool EXACT, int NC, bool DR1>
 static int launch_iteration(Ctx* c, const BatchPlan& plan, int fp_grid, int bp_grid, hipStream_t s) {
-  if (plan.fp_mode == 0)
+  if (plan.fp_mode == 4)
+    hipLaunchKernelGGL(k_fps_persist, dim3(fp_grid), dim3(FPS_RAYS), sizeof(float) * FPS_CS * FPS_MAXW, s,
+                       plan.args, plan.d_ctr);
+  else if (plan.fp_mode == 0)
     hipLaunchKernelGGL((k_fp_persist<EXACT, 1, false>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
   else if (plan.fp_mode == 1)
     hipLaunchKernelGGL((k_fp_persist<EXACT, 2, false>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
@@ -148,8 +183,10 @@ static int launch_iteration(Ctx* c, const BatchPlan& plan, int fp_grid, int bp_g
   else
     hipLaunchKernelGGL((k_fp_persist<EXACT, 2, true>), dim3(fp_grid), dim3(FP_BLOCK), 0, s, plan.args, plan.d_ctr);
   const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
-  hipLaunchKernelGGL((k_bp_persist<EXACT, NC, DR1, BATCH_PPT>), dim3(bp_grid), dim3(256), lds, s,
-                     plan.args, plan.d_ctr);
+  if (plan.bp_ppt == 4)
+    hipLaunchKernelGGL((k_bp_persist<EXACT, NC, DR1, 4>), dim3(bp_grid), dim3(256), lds, s, plan.args, plan.d_ctr);
+  else
+    hipLaunchKernelGGL((k_bp_persist<EXACT, NC, DR1, 2>), dim3(bp_grid), dim3(256), lds, s, plan.args, plan.d_ctr);
   return launched(c, "persistent SIRT kernels");
 }
 
@@ -209,8 +246,20 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   float* fws = (float*)((char*)ws + 256);
   BatchArgs& args = plan.args;
   args.n = count;
-  if (const char* m = getenv("SYNKRO_BATCH_FP_MODE")) plan.fp_mode = atoi(m) & 3;
-  const int rpt = (plan.fp_mode & 1) ? 2 : 1;
+  const bool ex = c->exact != 0;
+  bool staged_fits = !ex;
+  for (int i = 0; i < count; i++) {
+    const Geom& G = problems[i].geom->g;
+    staged_fits = staged_fits && G.fps_w > 0 && G.fps_w <= FPS_MAXW;
+  }
+  if (staged_fits) plan.fp_mode = 4;
+  if (const char* m = getenv("SYNKRO_BATCH_FP_MODE")) {
+    const int want = atoi(m);
+    plan.fp_mode = (want == 4 && !staged_fits) ? plan.fp_mode : (want == 4 ? 4 : (want & 3));
+  }
+  if (const char* m = getenv("SYNKRO_BATCH_BP_PPT")) plan.bp_ppt = atoi(m) == 2 ? 2 : 4;
+  const int ppt = plan.bp_ppt;
+  const int fp_item_bins = plan.fp_mode == 4 ? FPS_RAYS : 64 * ((plan.fp_mode & 1) ? 2 : 1);
   for (int oi = 0; oi < count; oi++) {
     const sk_problem& P = problems[order[oi]];
     const Geom& G = P.geom->g;
@@ -220,11 +269,11 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     fws += (size_t)P.batch * G.A * G.R;
     BatchProb B{};
     B.fp = fp_params(&G, FpArgs{P.d_img, imgT, q, P.d_sino, G.d_rowinv, P.batch});
-    B.bp = bp_params(&G, BpArgs{q, P.d_img, imgT, G.d_colinv, P.batch}, BATCH_PPT);
+    B.bp = bp_params(&G, BpArgs{q, P.d_img, imgT, G.d_colinv, P.batch}, ppt);
     B.batch = P.batch;
-    B.fp_wb = (G.R + 64 * rpt - 1) / (64 * rpt);
+    B.fp_wb = (G.R + fp_item_bins - 1) / fp_item_bins;
     B.bp_tx = (G.N + BP_TW - 1) / BP_TW;
-    B.bp_ty = (G.N + 16 * BATCH_PPT - 1) / (16 * BATCH_PPT);
+    B.bp_ty = (G.N + 16 * ppt - 1) / (16 * ppt);
     B.fp_begin = args.fp_total;
     B.bp_begin = args.bp_total;
     args.fp_total += (long)P.batch * G.A * B.fp_wb;
@@ -240,7 +289,6 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   if (sizeof(float2) * BP_LB * (size_t)plan.Wmax > 60 * 1024)
     return fail(c, SK_EINVAL, "backprojection window exceeds LDS");
 
-  const bool ex = c->exact != 0;
   const size_t lds = sizeof(float2) * BP_LB * (size_t)plan.Wmax;
   int fp_grid = 0, bp_grid = 0;
   // occupancy (and so the persistent grid) of the forward kernel that will run
@@ -248,16 +296,22 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
                            (const void*)k_fp_persist<true, 1, true>, (const void*)k_fp_persist<true, 2, true>};
   const void* kfp_fast[4] = {(const void*)k_fp_persist<false, 1, false>, (const void*)k_fp_persist<false, 2, false>,
                              (const void*)k_fp_persist<false, 1, true>, (const void*)k_fp_persist<false, 2, true>};
-  const void* kfp = ex ? kfp_ex[plan.fp_mode & 3] : kfp_fast[plan.fp_mode & 3];
-  const void* kbp =
-      plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, BATCH_PPT> : (const void*)k_bp_persist<false, 2, true, BATCH_PPT>)
-               : (ex ? (const void*)k_bp_persist<true, 0, false, BATCH_PPT> : (const void*)k_bp_persist<false, 0, false, BATCH_PPT>);
-  rc = persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
+  const void* kfp = plan.fp_mode == 4 ? (const void*)k_fps_persist
+                                      : (ex ? kfp_ex[plan.fp_mode & 3] : kfp_fast[plan.fp_mode & 3]);
+  const void* kbp2 =
+      plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, 2> : (const void*)k_bp_persist<false, 2, true, 2>)
+               : (ex ? (const void*)k_bp_persist<true, 0, false, 2> : (const void*)k_bp_persist<false, 0, false, 2>);
+  const void* kbp4 =
+      plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, 4> : (const void*)k_bp_persist<false, 2, true, 4>)
+               : (ex ? (const void*)k_bp_persist<true, 0, false, 4> : (const void*)k_bp_persist<false, 0, false, 4>);
+  const void* kbp = plan.bp_ppt == 4 ? kbp4 : kbp2;
+  rc = plan.fp_mode == 4 ? persistent_grid(c, kfp, FPS_RAYS, sizeof(float) * FPS_CS * FPS_MAXW, &fp_grid)
+                         : persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
   if (!rc) rc = persistent_grid(c, kbp, 256, lds, &bp_grid);
   if (rc) return rc;
   if (getenv("SYNKRO_VERBOSE"))
-    fprintf(stderr, "sk_sirt_batched: %d groups, fp items %ld (grid %d), bp items %ld (grid %d), W %d\n",
-            count, args.fp_total, fp_grid, args.bp_total, bp_grid, plan.Wmax);
+    fprintf(stderr, "sk_sirt_batched: %d groups, fp mode %d items %ld (grid %d), bp 16x%d tiles items %ld (grid %d), W %d\n",
+            count, plan.fp_mode, args.fp_total, fp_grid, 16 * plan.bp_ppt, args.bp_total, bp_grid, plan.Wmax);
 
   // Capture reset + iters x (fp, bp) once, launch once.
   hipStream_t cs = c->capture_stream;

@@ -668,4 +668,206 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
   }
 }
 
+// ------------------------------------------------------------------ staged forward projection
+// A workgroup = FPS_RAYS consecutive bins of one line.  The frame rows are walked in blocks of
+// FPS_ROWS: the waves load the block's image-row segment (the columns any of the workgroup's
+// rays can touch, at most FPS_MAXW) into LDS with coalesced dword loads, then every lane forms
+// its ray's row steps from LDS with the fast-mode step of fp_row_add.
+constexpr int FPS_RAYS = 256;
+constexpr int FPS_ROWS = 16;
+constexpr int FPS_CPL = 6;                       // staged floats per lane and row
+constexpr int FPS_MAXW = FPS_CPL * 64;           // widest segment (floats)
+constexpr int FPS_CS = FPS_ROWS + 1;             // LDS stride of a staged column (floats)
+
+typedef __attribute__((address_space(3))) float lds_float;
+
+template <int K>
+__device__ __forceinline__ int bcast16i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
+}
+
+template <bool POS, int NBUF>
+__device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
+                                          int i0w, int i1w, float slope, float icpt,
+                                          float icpt_lo, float* seg) {
+  const int tid = threadIdx.x;
+  const int l16 = tid & 15;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr int RPW = FPS_ROWS / (FPS_RAYS / 64);  // staged rows per wave
+  float st[RPW][FPS_CPL];
+  // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segments all start at
+  // image column sb, one left of floor(x) of the lowest ray over the block's rows (xbot is
+  // rounded from i*slope + icpt exactly as each lane rounds it, and rounding is monotone, so
+  // every lane's floor(xbot) - 1 lies at or right of it).
+  auto block_rows = [&](int ib, float& tab, int& sb) {
+    tab = (float)(ib + l16) * slope;
+    const int sv = floor_to_int(tab + icpt_lo) + h - 1;
+    // one start for the whole block: the lower of its first and last rows' (monotone in i)
+    sb = min(__builtin_amdgcn_readlane(sv, 0), __builtin_amdgcn_readlane(sv, FPS_ROWS - 1));
+  };
+  // Wave w stages rows RPW*w .. RPW*w + RPW-1 of the block, lane l columns s0 + l + 64m:
+  // coalesced dword loads through the row's descriptor (zero outside the image row and
+  // for rows past i1).
+  auto load = [&](int ib, int s0) {
+#pragma unroll
+    for (int t = 0; t < RPW; ++t) {
+      const int i = ib + RPW * wave + t;
+      const __amdgpu_buffer_rsrc_t rr = row_rsrc(src + (size_t)(i < i1 ? i + h : 0) * N, i < i1 ? N : 0);
+      const int vo = (s0 + lane) << 2;
+      if (s0 >= 0) {  // wave-uniform: non-negative offsets, the constant parts as immediates
+#pragma unroll
+        for (int m = 0; m < FPS_CPL; ++m)
+          st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, vo + 256 * m, 0, 0));
+      } else {
+        // segment starting left of the image: each load's full offset in its own register
+        // (a negative one is out of range as an unsigned offset and reads 0)
+#pragma unroll
+        for (int m = 0; m < FPS_CPL; ++m) {
+          int o = vo + 256 * m;
+          asm volatile("" : "+v"(o));
+          st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, o, 0, 0));
+        }
+      }
+    }
+  };
+  // LDS layout of a staged block: column-interleaved, column j of row slot r at j*CS + r
+  // (CS = FPS_ROWS + 1 spreads consecutive columns over the banks), so a lane's pair in row K
+  // is one address plus the constant offsets K and K + CS.
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < RPW; ++t) {
+      float* base = seg + (size_t)buf * FPS_MAXW * FPS_CS + lane * FPS_CS + RPW * wave + t;
+#pragma unroll
+      for (int m = 0; m < FPS_CPL; ++m) base[64 * FPS_CS * m] = st[t][m];
+    }
+  };
+  float acc = 0.0f;
+  const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
+  // The LDS reads run eight rows ahead of the sums (rolling window of eight row registers).
+  float xbs[8], vas[8], vcs[8];
+  auto fetch = [&](auto kc, float tab, int bbase) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int S = K & 7;
+    xbs[S] = bcast16<K>(tab) + icpt;
+    // LDS byte address of the pair's first column in row slot 0
+    const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * FPS_CS) + bbase);
+    const lds_float* p = (const lds_float*)(uintptr_t)off;
+    vas[S] = p[K];
+    vcs[S] = p[K + FPS_CS];
+  };
+  auto sum_row = [&](int S) {
+    const float xb = xbs[S], xt = xb + slope;
+    acc = POS ? fp_row_add<true>(acc, xb, xt, vas[S], vcs[S]) : fp_row_add<false>(acc, xb, xt, vcs[S], vas[S]);
+  };
+  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + 8
+    constexpr int K = decltype(kc)::value;
+    sum_row(K);
+    fetch(std::integral_constant<int, K + 8>{}, tab, bbase);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto compute = [&](int buf, float tab, int sb) {
+    // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
+    const int bbase = (buf * FPS_MAXW * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
+    fetch(std::integral_constant<int, 0>{}, tab, bbase);
+    fetch(std::integral_constant<int, 1>{}, tab, bbase);
+    fetch(std::integral_constant<int, 2>{}, tab, bbase);
+    fetch(std::integral_constant<int, 3>{}, tab, bbase);
+    fetch(std::integral_constant<int, 4>{}, tab, bbase);
+    fetch(std::integral_constant<int, 5>{}, tab, bbase);
+    fetch(std::integral_constant<int, 6>{}, tab, bbase);
+    fetch(std::integral_constant<int, 7>{}, tab, bbase);
+    __builtin_amdgcn_sched_barrier(0);
+    step(std::integral_constant<int, 0>{}, tab, bbase);
+    step(std::integral_constant<int, 1>{}, tab, bbase);
+    step(std::integral_constant<int, 2>{}, tab, bbase);
+    step(std::integral_constant<int, 3>{}, tab, bbase);
+    step(std::integral_constant<int, 4>{}, tab, bbase);
+    step(std::integral_constant<int, 5>{}, tab, bbase);
+    step(std::integral_constant<int, 6>{}, tab, bbase);
+    step(std::integral_constant<int, 7>{}, tab, bbase);
+#pragma unroll
+    for (int S = 0; S < 8; ++S) sum_row(S);
+  };
+  float tab, tabn;
+  int sb, sbn;
+  block_rows(i0, tab, sb);
+  load(i0, sb);
+  store(0);
+  __syncthreads();
+  int b = 0;
+  for (int ib = i0; ib < i1; ib += FPS_ROWS, ++b) {  // i0, i1 uniform: every lane reaches each barrier
+    const bool more = ib + FPS_ROWS < i1;
+    if (more) {
+      block_rows(ib + FPS_ROWS, tabn, sbn);
+      load(ib + FPS_ROWS, sbn);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next block's loads stay ahead of the compute
+    // A wave skips blocks outside its own rays' row range: there they add exactly 0.
+    if (ib + FPS_ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sb);
+    if constexpr (NBUF == 1) __syncthreads();
+    if (more) store(NBUF == 2 ? ((b + 1) & 1) : 0);
+    __syncthreads();
+    tab = tabn;
+    sb = sbn;
+  }
+  return acc;
+}
+
+// One staged work item: bins blk*FPS_RAYS .. +FPS_RAYS-1 of line j of image bz, one ray per
+// thread of a FPS_RAYS-thread workgroup.  seg: NBUF * FPS_CS * FPS_MAXW floats of LDS.  All
+// barriers are workgroup-uniform; on return every wave has passed the walk's last barrier,
+// so the caller may reuse seg at once.
+template <int MODE, int NBUF>
+__device__ __forceinline__ void fps_item(const FpParams& P, int j, int blk, int bz, float* seg) {
+  const int first = blk * FPS_RAYS;
+  const int r = first + (int)threadIdx.x;
+  const Line L = P.lines[j];
+  const int N = P.N, h = N >> 1;
+  const float cost = L.cost, slope = L.slope;
+  const float icpt = (P.rho0 + (float)r * P.drho) / cost;  // forwardprojection.fut:33-34
+  // The workgroup's intercepts are monotone in r: the extremes are lanes 0 and FPS_RAYS-1.
+  const float ia = (P.rho0 + (float)first * P.drho) / cost;
+  const float iz = (P.rho0 + (float)(first + FPS_RAYS - 1) * P.drho) / cost;
+  const float icpt_lo = fminf(ia, iz);
+  int i0, i1, i0w, i1w;
+  {
+    // the workgroup's row range (hull over its rays) and the wave's own
+    int a0, a1, b0, b1;
+    const int last = min(first + FPS_RAYS - 1, P.R - 1);
+    row_clip(slope, ia, h, a0, a1);
+    row_clip(slope, (P.rho0 + (float)last * P.drho) / cost, h, b0, b1);
+    i0 = __builtin_amdgcn_readfirstlane(min(a0, b0));
+    i1 = __builtin_amdgcn_readfirstlane(max(a1, b1));
+    const int wfirst = first + (int)(threadIdx.x & ~63u);
+    const int wlast = min(wfirst + 63, P.R - 1);
+    if (wfirst > wlast) {
+      i0w = i1w = i0;  // no valid ray in this wave
+    } else {
+      row_clip(slope, (P.rho0 + (float)wfirst * P.drho) / cost, h, a0, a1);
+      row_clip(slope, (P.rho0 + (float)wlast * P.drho) / cost, h, b0, b1);
+      i0w = min(a0, b0);
+      i1w = max(a1, b1);
+    }
+    i0w = __builtin_amdgcn_readfirstlane(i0w);
+    i1w = __builtin_amdgcn_readfirstlane(i1w);
+  }
+  const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+  float acc = 0.0f;
+  if (i0 < i1) {
+    if (L.flags & LINE_SLOPE_POS)
+      acc = fps_walk<true, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+    else
+      acc = fps_walk<false, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+  }
+  if (r >= P.R) return;  // after the walk: every lane took part in its barriers
+  const float s = acc * L.lbase;
+  const size_t o = (size_t)bz * P.A * P.R + (size_t)L.orig * P.R + r;
+  if constexpr (MODE == FP_PLAIN)
+    P.out[o] = s;
+  else
+    P.out[o] = P.rowinv[(size_t)L.orig * P.R + r] * (P.b[o] - s);  // SIRT.fut:33-36
+}
+
+
 }  // namespace sk
