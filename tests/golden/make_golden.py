@@ -10,6 +10,10 @@ Writes
                    SHA-256 of the full float32 outputs).
   c1.npz           the 64x64 / 90-angle / 91-bin configuration (10-ellipse phantom, seed
                    0): angles, phantom, and the oracle's fp, bp and 10-iteration SIRT.
+  ref_results_rtx2080ti/{fut,astra}_{fp,bp}.csv
+                   the reference's own timing results (output/20190523_065504: Futhark
+                   and ASTRA on an RTX 2080 Ti, sizes 64..4096, 10 runs each, size;us),
+                   copied unchanged for synkrotomo_amd.sweep --compare.
 The expected outputs come from the oracle (oracle/oracle.c), the CPU restatement of
 the reference; the reference itself (Futhark) cannot be built here.  Reading the
 reference's input file needs /root/reference; the outputs are committed so that nothing
@@ -31,6 +35,7 @@ import oracle  # noqa: E402
 from synkrotomo_amd.phantom import ellipse_phantom  # noqa: E402
 
 REF_FIXTURE = "/root/reference/futhark/data/fpinputf32rad512"
+REF_RESULTS = "/root/reference/output/20190523_065504"
 
 
 def parse_futhark_values(text: str):
@@ -102,7 +107,16 @@ def make_c1():
     print("c1: fp sha", sha(fp)[:12], "sirt10 sha", sha(sirt10)[:12])
 
 
+def copy_ref_results():
+    out = os.path.join(HERE, "ref_results_rtx2080ti")
+    os.makedirs(out, exist_ok=True)
+    for name in ("fut_fp.csv", "fut_bp.csv", "astra_fp.csv", "astra_bp.csv"):
+        with open(os.path.join(REF_RESULTS, name)) as f, open(os.path.join(out, name), "w") as g:
+            g.write(f.read())
+
+
 if __name__ == "__main__":
     oracle.build()
     make_c1()
     make_fixture512()
+    copy_ref_results()
