@@ -220,6 +220,25 @@ int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);
 void sk_mat_destroy(sk_mat* m);
 
+/* ---------------------------------------------------------------- measured data */
+/* No reference counterpart: the reference reconstructs from sinograms that are already line
+ * integrals (its SIRT benchmark reads data/tooth.in, futhark/SIRT.fut:1-2).
+ * Flat/dark-field normalisation and -log of raw detector counts [batch][A][R]:
+ *   dark[r] = mean of the n_dark dark frames d_dark[n_dark][R] (0 if n_dark == 0),
+ *   flat[r] = mean of the n_flat open-beam frames d_flat[n_flat][R] (both summed in FP64),
+ *   T = (raw - dark[r]) / (flat[r] - dark[r]),
+ *   out = -log(max(T, min_transmission)) / scale.
+ * A bin whose flat mean does not exceed its dark mean reads T = 0 (fully absorbing); a NaN
+ * count reads T = min_transmission. 
+ * d_out may equal d_raw. */
+int sk_normalize(sk_ctx* ctx, int batch, int A, int R, const float* d_raw, int n_flat,
+                 const float* d_flat, int n_dark, const float* d_dark, float min_transmission,
+                 float scale, float* d_out, void* stream);
+/* Centre of mass of each of `rows` sinogram rows of R bins: d_com[i] = sum_r r*v / sum_r v in
+ * FP64 (NaN where sum_r v == 0).  The rotation axis sits at the bin c0 of the fit
+ * com(theta) = c0 + p*cos(theta) + q*sin(theta); rhozero = -c0*deltarho centres a geometry. */
+int sk_row_centroid(sk_ctx* ctx, int64_t rows, int R, const float* d_sino, double* d_com, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

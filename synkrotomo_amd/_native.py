@@ -63,6 +63,8 @@ SIGNATURES = {
     "sk_mat_set_spmv_variant": (_i, [_vp, _i]),
     "sk_spmv": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_mat_destroy": (None, [_vp]),
+    "sk_normalize": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp, _f, _f, _vp, _vp]),
+    "sk_row_centroid": (_i, [_vp, ctypes.c_int64, _i, _vp, _vp, _vp]),
 }
 
 

@@ -50,14 +50,19 @@ struct Geom {
   int fp_variant = -1;        // tuned kernel variants (-1: the context's default)
   int bp_variant = -1;
   int device = 0;
-  // Cached SIRT iteration graph and the arguments it was captured with.
-  hipGraphExec_t sirt_exec = nullptr;
-  struct {
+  // Cached SIRT iteration graphs and the arguments each was captured with (a few, so that
+  // callers alternating between buffer sets -- double-buffered streaming -- reuse them).
+  struct SirtGraph {
+    hipGraphExec_t exec = nullptr;
     const float* img = nullptr;
     const float* b = nullptr;
     const void* ws = nullptr;
-    int batch = 0, exact = -1, iters = 0;
-  } sirt_key;
+    int batch = 0, exact = -1, iters = 0, fpv = -2, bpv = -2;
+    unsigned long long used = 0;  // last use (LRU eviction)
+  };
+  static constexpr int SIRT_GRAPHS = 4;
+  SirtGraph sirt_graphs[SIRT_GRAPHS];
+  unsigned long long sirt_clock = 0;
 };
 
 struct Ctx {

@@ -211,7 +211,8 @@ void sk_geom_destroy(sk_geom* g) {
   if (!g) return;
   (void)hipSetDevice(g->g.device);
   (void)hipDeviceSynchronize();
-  if (g->g.sirt_exec) (void)hipGraphExecDestroy(g->g.sirt_exec);
+  for (auto& sg : g->g.sirt_graphs)
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   if (g->g.d_lines) (void)hipFree(g->g.d_lines);
   if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
   if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);

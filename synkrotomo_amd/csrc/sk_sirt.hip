@@ -36,11 +36,6 @@ static int sirt_iteration(Ctx* c, const Geom* g, int batch, float* img, float* i
   return launch_bp(c, g, BP_SIRT, W_REF, ba, s);
 }
 
-static void drop_graph(Geom* g) {
-  if (g->sirt_exec) (void)hipGraphExecDestroy(g->sirt_exec);
-  g->sirt_exec = nullptr;
-  g->sirt_key = {};
-}
 
 // Capture `k` iterations into an executable graph.
 static int capture(Ctx* c, Geom* g, int k, int batch, float* img, float* imgT, const float* b,
@@ -110,26 +105,39 @@ int sk_sirt(sk_ctx* ctx, sk_geom* g, int batch, float* d_img, const float* d_sin
   if (rc) return rc;
 
   const int K = iters < 8 ? iters : 8;
-  const bool hit = G.sirt_exec && G.sirt_key.img == d_img && G.sirt_key.b == d_sino &&
-                   G.sirt_key.ws == ws && G.sirt_key.batch == batch &&
-                   G.sirt_key.exact == c->exact && G.sirt_key.iters == K;
-  if (!hit) {
-    if (G.sirt_exec) {
-      SK_HIP(c, hipStreamSynchronize(s));
-      drop_graph(&G);
+  const int fpv = G.fp_variant >= 0 ? G.fp_variant : c->fp_variant;
+  const int bpv = G.bp_variant >= 0 ? G.bp_variant : c->bp_variant;
+  Geom::SirtGraph* slot = nullptr;
+  for (auto& sg : G.sirt_graphs)
+    if (sg.exec && sg.img == d_img && sg.b == d_sino && sg.ws == ws && sg.batch == batch &&
+        sg.exact == c->exact && sg.iters == K && sg.fpv == fpv && sg.bpv == bpv)
+      slot = &sg;
+  if (!slot) {
+    // an empty slot, else the least recently used one (its graph may still run, on any
+    // stream: wait for the device)
+    slot = &G.sirt_graphs[0];
+    for (auto& sg : G.sirt_graphs)
+      if (!sg.exec || (slot->exec && sg.used < slot->used)) slot = &sg;
+    if (slot->exec) {
+      SK_HIP(c, hipDeviceSynchronize());
+      (void)hipGraphExecDestroy(slot->exec);
+      *slot = Geom::SirtGraph{};
     }
     hipGraphExec_t ex = nullptr;
     rc = capture(c, &G, K, batch, d_img, imgT, d_sino, q, &ex);
     if (rc) return rc;
-    G.sirt_exec = ex;
-    G.sirt_key.img = d_img;
-    G.sirt_key.b = d_sino;
-    G.sirt_key.ws = ws;
-    G.sirt_key.batch = batch;
-    G.sirt_key.exact = c->exact;
-    G.sirt_key.iters = K;
+    slot->exec = ex;
+    slot->img = d_img;
+    slot->b = d_sino;
+    slot->ws = ws;
+    slot->batch = batch;
+    slot->exact = c->exact;
+    slot->iters = K;
+    slot->fpv = fpv;
+    slot->bpv = bpv;
   }
-  for (int done = 0; done + K <= iters; done += K) SK_HIP(c, hipGraphLaunch(G.sirt_exec, s));
+  slot->used = ++G.sirt_clock;
+  for (int done = 0; done + K <= iters; done += K) SK_HIP(c, hipGraphLaunch(slot->exec, s));
   for (int it = 0; it < iters % K; it++) {
     rc = sirt_iteration(c, &G, batch, d_img, imgT, d_sino, G.d_rowinv, G.d_colinv, q, s);
     if (rc) return rc;

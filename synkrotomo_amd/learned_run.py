@@ -200,6 +200,10 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         for tname, target in (("1e-2", np.full(B, 1e-2)), ("1.1x_floor", 1.1 * floor)):
             kp = _first_hit(res_p, target)
             kl = _first_hit(res_l, target)
+            # image error plain SIRT has when it reaches the target, and the first learned
+            # state (after the stages, then after each extra plain iteration) at or below it
+            e_at = np.where(kp > 0, err_p[np.maximum(kp, 1) - 1, np.arange(B)], -1.0)
+            ke = _first_hit(err_l, e_at)
             for i in range(B):
                 rows.append({"target": tname, "floor": float(floor[i]), "fell_back": bool(fell[i]),
                              "plain_iters": int(kp[i]),
@@ -209,7 +213,8 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                              "plain_res_at_max": float(res_p[-1, i]),
                              "learned_res_after_stages": float(res_l[0, i]),
                              "learned_err_after_stages": float(err_l[0, i]),
-                             "plain_err_at_stages": float(err_p[stages - 1, i])})
+                             "plain_err_at_stages": float(err_p[stages - 1, i]),
+                             "learned_extra_iters_to_plain_err": int(ke[i] - 1) if kp[i] > 0 and ke[i] > 0 else -1})
         if verbose:
             print(json.dumps({"done": start + B}), flush=True)
 
@@ -236,6 +241,16 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                 "median_learned_extra_iters": float(np.median([r["learned_extra_iters"] for r in both])),
                 "median_plain_err": round(float(np.median([r["plain_err"] for r in both])), 5),
                 "median_learned_err": round(float(np.median([r["learned_err"] for r in both])), 5)})
+            # the same image error as plain SIRT at its target, reached by the learned method
+            eq = [r for r in rs if r["plain_iters"] > 0 and r["learned_extra_iters_to_plain_err"] >= 0]
+            if eq:
+                tpe = np.array([r["plain_iters"] * t_iter for r in eq])
+                tle = np.array([t_recon + r["learned_extra_iters_to_plain_err"] * t_iter for r in eq])
+                summary["equal_image_error"] = {
+                    "of": len(eq), "median_learned_extra_iters": float(np.median(
+                        [r["learned_extra_iters_to_plain_err"] for r in eq])),
+                    "median_learned_ms": round(float(np.median(tle)) * 1e3, 3),
+                    "speedup_median": round(float(np.median(tpe) / np.median(tle)), 2)}
         out[tname] = summary
     out["fell_back"] = sum(r["fell_back"] for r in rows if r["target"] == "1e-2")
     rs = [r for r in rows if r["target"] == "1e-2"]
