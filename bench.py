@@ -111,6 +111,71 @@ def sirt_leg(geo, x_true, sino, iters: int = 50) -> dict:
             "residual": round(res, 6), "image_rel_err": round(err, 6)}
 
 
+def stream3d_leg(ctx, S: int = 512, N: int = 512, A: int = 512, iters: int = 10, chunk: int = 64) -> dict:
+    """Out-of-core SIRT3D: S slices of N^2 (A angles, centred unit bins) from host memory in
+    chunks of ``chunk`` slices over three streams, against the same SIRT with the stack
+    resident (compute only) and against upload + SIRT + download in series."""
+    import numpy as np
+    import torch
+    from synkrotomo_amd.autotune import tuned
+    from synkrotomo_amd.projector import centred_bins, uniform_angles
+    from synkrotomo_amd.stream3d import sirt3d_streamed
+    R, rz = centred_bins(N)
+    geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
+    tuned(geo)
+    proj = np.random.default_rng(5).random((S, A, R), dtype=np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d = torch.from_numpy(proj).cuda()
+    x = torch.zeros(S, N, N, device="cuda")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    geo.sirt_(x, d, iters)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ref = x.cpu().numpy()
+    t3 = time.perf_counter()
+    x.zero_()
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    geo.sirt_(x, d, iters)                      # warm: the compute-only time
+    torch.cuda.synchronize()
+    compute = time.perf_counter() - t4
+    del d, x
+    torch.cuda.empty_cache()
+    sirt3d_streamed(geo, proj[: 2 * chunk], iters, chunk)   # graphs for both buffer sets
+    t5 = time.perf_counter()
+    out = sirt3d_streamed(geo, proj, iters, chunk)
+    streamed = time.perf_counter() - t5
+    return {"slices": S, "N": N, "angles": A, "bins": R, "iterations": iters, "chunk": chunk,
+            "streamed_s": round(streamed, 4), "resident_compute_s": round(compute, 4),
+            "serial_upload_sirt_download_s": round((t1 - t0) + (t2 - t1) + (t3 - t2), 4),
+            "bitwise_equal_to_resident": bool(np.array_equal(out, ref))}
+
+
+def normalize_leg(ctx, A: int = 2048, R: int = 2896, batch: int = 16, reps: int = 20) -> dict:
+    """Flat/dark normalisation and -log of a [batch, A, R] count stack (16 frames each of flat
+    and dark): HBM-bound, 8 bytes per entry (read raw, write out)."""
+    import torch
+    from synkrotomo_amd.preprocess import normalize
+    g = torch.Generator(device="cuda").manual_seed(1)
+    raw = torch.rand((batch, A, R), device="cuda", generator=g) * 1000 + 10
+    flat = torch.rand((16, R), device="cuda", generator=g) * 10 + 1000
+    dark = torch.rand((16, R), device="cuda", generator=g) * 5
+    out = torch.empty_like(raw)
+    normalize(raw, flat, dark, out=out, ctx=ctx)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        normalize(raw, flat, dark, out=out, ctx=ctx)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 8.0 * raw.numel() / (ms * 1e-3) / 1e9
+    return {"shape": [batch, A, R], "ms": round(ms, 4), "achieved_GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def _batched_case(ctx, specs, iters: int, seed: int) -> dict:
     """specs: [(N, angles, images)], centred unit bins.  Batched SIRT (one persistent launch
     sequence, one hipGraph) against the same problems group by group through sk_sirt with
@@ -231,11 +296,18 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-SIRT leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-SIRT leg")
     ap.add_argument("--no-sirt", action="store_true", help="skip the 2048^2 SIRT leg")
+    ap.add_argument("--no-stream", action="store_true", help="skip the out-of-core SIRT3D leg")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="only the dense fp/bp steps (every side leg and the CPU baseline skipped)")
+    ap.add_argument("--no-prep", action="store_true", help="skip the normalisation leg")
     ap.add_argument("--batched-iters", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use default kernel variants")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device copy first (reference dispatch for HBM counters)")
     args = ap.parse_args()
+    if args.dense_only:
+        for leg in ("cpu", "sparse", "batched", "learned", "sirt", "stream", "prep"):
+            setattr(args, f"no_{leg}", True)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the projector shards by "
               "angle across GPUs, which this build does not run.", file=sys.stderr)
@@ -319,12 +391,18 @@ def main() -> int:
                      "alg_bytes_per_launch": alg_bytes()},
         "cpu_baseline": None,
         "sirt": None,
+        "stream3d": None,
+        "normalize": None,
         "sparse": None,
         "batched": None,
         "learned": None,
     }
     if not args.no_sirt:
         out["sirt"] = sirt_leg(geo, x, sino)
+    if not args.no_stream:
+        out["stream3d"] = stream3d_leg(ctx)
+    if not args.no_prep:
+        out["normalize"] = normalize_leg(ctx)
     if not args.no_learned:
         out["learned"] = learned_leg(ctx)
     if not args.no_batched:

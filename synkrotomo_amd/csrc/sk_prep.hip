@@ -36,8 +36,9 @@ __global__ __launch_bounds__(256) void k_field_means(int R, int nf, int nd, cons
   fd[R + r] = range > 0.0f ? 1.0f / range : 0.0f;
 }
 
-// Grid-stride over [rows][R]; a row is one (image, angle).  Four consecutive entries per
-// thread and step when R is a multiple of 4 (16-byte loads and stores).
+// Grid (chunks of a row, rows): thread t of block (bx, by) handles entry (or 4-entry group)
+// bx*256 + t of rows by, by + gridDim.y, ...  Four consecutive entries per thread when R is a
+// multiple of 4 and both arrays are 16-byte aligned (16-byte loads and stores).
 // raw and out may be the same array (each entry is read, then written, by one thread).
 template <bool VEC4>
 __global__ __launch_bounds__(256) void k_normalize(long rows, int R, const float* raw,
@@ -49,19 +50,25 @@ __global__ __launch_bounds__(256) void k_normalize(long rows, int R, const float
     const float t = (v - dark[r]) * irange[r];
     return -logf(fmaxf(t, tmin)) * inv_scale;  // fmaxf(NaN, tmin) = tmin
   };
+  const int q = blockIdx.x * 256 + threadIdx.x;
   if constexpr (VEC4) {
-    const long n4 = rows * (long)(R >> 2);
-    const float4* in4 = (const float4*)raw;
-    float4* out4 = (float4*)out;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long)gridDim.x * 256) {
-      const int r = (int)(i % (R >> 2)) << 2;
-      const float4 v = in4[i];
-      out4[i] = make_float4(one(v.x, r), one(v.y, r + 1), one(v.z, r + 2), one(v.w, r + 3));
+    const int R4 = R >> 2;
+    if (q >= R4) return;
+    const int r = q << 2;
+    const float d0 = dark[r], d1 = dark[r + 1], d2 = dark[r + 2], d3 = dark[r + 3];
+    const float i0 = irange[r], i1 = irange[r + 1], i2 = irange[r + 2], i3 = irange[r + 3];
+    for (long row = blockIdx.y; row < rows; row += gridDim.y) {
+      const float4 v = reinterpret_cast<const float4*>(raw + row * R)[q];
+      float4 o;
+      o.x = -logf(fmaxf((v.x - d0) * i0, tmin)) * inv_scale;
+      o.y = -logf(fmaxf((v.y - d1) * i1, tmin)) * inv_scale;
+      o.z = -logf(fmaxf((v.z - d2) * i2, tmin)) * inv_scale;
+      o.w = -logf(fmaxf((v.w - d3) * i3, tmin)) * inv_scale;
+      reinterpret_cast<float4*>(out + row * R)[q] = o;
     }
   } else {
-    const long n = rows * (long)R;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-      out[i] = one(raw[i], (int)(i % R));
+    if (q >= R) return;
+    for (long row = blockIdx.y; row < rows; row += gridDim.y) out[row * R + q] = one(raw[row * R + q], q);
   }
 }
 
@@ -111,13 +118,13 @@ int sk_normalize(sk_ctx* ctx, int batch, int A, int R, const float* d_raw, int n
   hipLaunchKernelGGL(k_field_means, dim3((R + 255) / 256), dim3(256), 0, s, R, n_flat, n_dark, d_flat, d_dark, fd);
   const long rows = (long)batch * A;
   const bool vec4 = (R % 4 == 0) && ((uintptr_t)d_raw % 16 == 0) && ((uintptr_t)d_out % 16 == 0);
-  const long work = vec4 ? rows * (R / 4) : rows * (long)R;
-  const int blocks = (int)std::min<long>((work + 255) / 256, 8192);
+  const int per_row = vec4 ? R / 4 : R;
+  const dim3 grid((unsigned)((per_row + 255) / 256), (unsigned)std::min<long>(rows, 65535));
   if (vec4)
-    hipLaunchKernelGGL(k_normalize<true>, dim3(blocks), dim3(256), 0, s, rows, R, d_raw, fd, min_transmission,
+    hipLaunchKernelGGL(k_normalize<true>, grid, dim3(256), 0, s, rows, R, d_raw, fd, min_transmission,
                        1.0f / scale, d_out);
   else
-    hipLaunchKernelGGL(k_normalize<false>, dim3(blocks), dim3(256), 0, s, rows, R, d_raw, fd, min_transmission,
+    hipLaunchKernelGGL(k_normalize<false>, grid, dim3(256), 0, s, rows, R, d_raw, fd, min_transmission,
                        1.0f / scale, d_out);
   return launched(c, "normalisation kernels");
 }

@@ -27,14 +27,20 @@
 // a time; B: 64 lines at a time); ballots and mbcnt place each lane's entries, so the
 // entries come out in the sequential order above and the writes are contiguous.
 #include <cstdlib>
+#include <vector>
 
 #include "sk_common.h"
 
 struct sk_mat {
   int kind = 0;
   int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
-                          // 16-byte streams; -1: not yet timed
+                          // 16-byte streams; 3, 4: as 0, 2 with the gathers of steep rays
+                          // from x^T (SK_MAT_FP only); -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
+  // SK_MAT_FP: per original angle, 1 if its line is steep (its rays step through image rows:
+  // in x^T their pixels are contiguous), and the geometry's R and N
+  unsigned char* steep = nullptr;
+  int R = 0, N = 0;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
   float* val = nullptr;       // [nnz]
@@ -260,16 +266,29 @@ __global__ __launch_bounds__(256) void k_scan_add(int64_t* __restrict__ out, lon
 }
 
 // ------------------------------------------------------------------ SpMV: one wave per row
-template <int UNROLL>
-__global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ col,
-                                              const float* __restrict__ val, long rows, long ncols,
-                                              const float* __restrict__ x, float* __restrict__ y) {
-  const int lane = lane_id();
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= rows) return;
-  const float* xb = x + (size_t)blockIdx.y * ncols;
-  const long beg = rowptr[row], end = rowptr[row + 1];
+// Gathers of one row: from x itself, or (TR: a steep ray of A, whose pixels p = y*N + x step
+// through image rows) from x^T at x*N + y, where they are contiguous.  x^T holds exactly x's
+// values, so both read the same numbers: results are bitwise equal.
+struct SpGather {
+  const float* x;
+  int N;
+  double invN;  // 1/N
+  template <bool TR>
+  __device__ __forceinline__ float at(int c) const {
+    if constexpr (!TR) {
+      return x[c];
+    } else {
+      // y = floor(c / N): (c + 0.5)/N is at least 0.5/N away from an integer, far beyond the
+      // FP64 product's rounding for c < 2^31
+      const int yq = (int)(((double)c + 0.5) * invN);
+      return x[(c - yq * N) * N + yq];
+    }
+  }
+};
+
+template <int UNROLL, bool TR>
+__device__ __forceinline__ float spmv_row(const int32_t* __restrict__ col, const float* __restrict__ val,
+                                          long beg, long end, int lane, const SpGather& g) {
   float acc = 0.0f;
   long e = beg + lane;
   for (; e + 64 * (UNROLL - 1) < end; e += 64 * UNROLL) {
@@ -281,9 +300,31 @@ __global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr
       v[u] = __builtin_nontemporal_load(val + e + 64 * u);
     }
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], xb[c[u]], acc);
+    for (int u = 0; u < UNROLL; ++u) acc = fmaf(v[u], g.at<TR>(c[u]), acc);
   }
-  for (; e < end; e += 64) acc = fmaf(__builtin_nontemporal_load(val + e), xb[__builtin_nontemporal_load(col + e)], acc);
+  for (; e < end; e += 64)
+    acc = fmaf(__builtin_nontemporal_load(val + e), g.at<TR>(__builtin_nontemporal_load(col + e)), acc);
+  return acc;
+}
+
+// TR: steep rows (steep[row / R]) gather from xT; the others from x.
+template <int UNROLL, bool TR>
+__global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ col,
+                                              const float* __restrict__ val, long rows, long ncols,
+                                              const float* __restrict__ x, float* __restrict__ y,
+                                              const float* __restrict__ xT, const unsigned char* __restrict__ steep,
+                                              int R, int N, double invN) {
+  const int lane = lane_id();
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const size_t boff = (size_t)blockIdx.y * ncols;
+  const long beg = rowptr[row], end = rowptr[row + 1];
+  float acc;
+  if (TR && steep[row / R])  // wave-uniform
+    acc = spmv_row<UNROLL, TR>(col, val, beg, end, lane, SpGather{xT + boff, N, invN});
+  else
+    acc = spmv_row<UNROLL, false>(col, val, beg, end, lane, SpGather{x + boff, N, invN});
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
 }
@@ -292,21 +333,39 @@ __global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ rowptr
 // As k_spmv, but each lane streams aligned 4-entry chunks (16-byte column and value loads)
 // and keeps U chunks -- 4U gathers -- in flight per iteration; the at most 3 entries before
 // the first aligned chunk and after the last are handled by single lanes.
-template <int U>
+template <int U, bool TR>
+__device__ __forceinline__ float spmv_row_wide(const int32_t* __restrict__ col, const float* __restrict__ val,
+                                               long beg, long end, int lane, const SpGather& xb);
+
+template <int U, bool TR>
 __global__ __launch_bounds__(256) void k_spmv_wide(const int64_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ col,
                                                    const float* __restrict__ val, long rows,
                                                    long ncols, const float* __restrict__ x,
-                                                   float* __restrict__ y) {
+                                                   float* __restrict__ y, const float* __restrict__ xT,
+                                                   const unsigned char* __restrict__ steep, int R, int N,
+                                                   double invN) {
   const int lane = lane_id();
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= rows) return;
-  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const size_t boff = (size_t)blockIdx.y * ncols;
   const long beg = rowptr[row], end = rowptr[row + 1];
+  float acc;
+  if (TR && steep[row / R])  // wave-uniform
+    acc = spmv_row_wide<U, TR>(col, val, beg, end, lane, SpGather{xT + boff, N, invN});
+  else
+    acc = spmv_row_wide<U, false>(col, val, beg, end, lane, SpGather{x + boff, N, invN});
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
+template <int U, bool TR>
+__device__ __forceinline__ float spmv_row_wide(const int32_t* __restrict__ col, const float* __restrict__ val,
+                                               long beg, long end, int lane, const SpGather& xb) {
   const long a0 = min((beg + 3) & ~3L, end);  // first aligned entry (or end)
   const long a1 = max(a0, end & ~3L);         // end of the aligned body
   float acc = 0.0f;
-  if (beg + lane < a0) acc = fmaf(val[beg + lane], xb[col[beg + lane]], acc);
+  if (beg + lane < a0) acc = fmaf(val[beg + lane], xb.at<TR>(col[beg + lane]), acc);
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   typedef float f32x4 __attribute__((ext_vector_type(4)));
   const i32x4* c4 = reinterpret_cast<const i32x4*>(col);
@@ -323,15 +382,14 @@ __global__ __launch_bounds__(256) void k_spmv_wide(const int64_t* __restrict__ r
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      acc = fmaf(v[u].x, xb[c[u].x], acc);
-      acc = fmaf(v[u].y, xb[c[u].y], acc);
-      acc = fmaf(v[u].z, xb[c[u].z], acc);
-      acc = fmaf(v[u].w, xb[c[u].w], acc);
+      acc = fmaf(v[u].x, xb.at<TR>(c[u].x), acc);
+      acc = fmaf(v[u].y, xb.at<TR>(c[u].y), acc);
+      acc = fmaf(v[u].z, xb.at<TR>(c[u].z), acc);
+      acc = fmaf(v[u].w, xb.at<TR>(c[u].w), acc);
     }
   }
-  if (a1 + lane < end) acc = fmaf(val[a1 + lane], xb[col[a1 + lane]], acc);
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if (lane == 0) y[(size_t)blockIdx.y * rows + row] = acc;
+  if (a1 + lane < end) acc = fmaf(val[a1 + lane], xb.at<TR>(col[a1 + lane]), acc);
+  return acc;
 }
 
 // counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
@@ -429,6 +487,7 @@ static void mat_free(sk_mat* m) {
   if (m->rowptr) (void)hipFree(m->rowptr);
   if (m->col) (void)hipFree(m->col);
   if (m->val) (void)hipFree(m->val);
+  if (m->steep) (void)hipFree(m->steep);
   delete m;
 }
 
@@ -451,7 +510,20 @@ int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stre
   m->device = c->device;
   m->rows = which == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
   m->cols = which == SK_MAT_FP ? (long)G.N * G.N : (long)G.A * G.R;
-  hipError_t e = hipMalloc(&m->rowptr, sizeof(int64_t) * (size_t)(m->rows + 1));
+  m->R = G.R;
+  m->N = G.N;
+  hipError_t e = hipSuccess;
+  if (which == SK_MAT_FP) {
+    std::vector<unsigned char> st((size_t)G.A, 0);
+    for (const Line& L : G.lines) st[(size_t)L.orig] = (L.flags & LINE_FLAT) ? 0 : 1;
+    e = hipMalloc(&m->steep, st.size());
+    if (e == hipSuccess) e = hipMemcpy(m->steep, st.data(), st.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      mat_free(m);
+      return hip_fail(c, e, "steep-angle table");
+    }
+  }
+  e = hipMalloc(&m->rowptr, sizeof(int64_t) * (size_t)(m->rows + 1));
   if (e != hipSuccess) {
     mat_free(m);
     return hip_fail(c, e, "row pointer allocation");
@@ -517,7 +589,8 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 2) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 4) return SK_EINVAL;
+  if (variant >= 3 && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -565,32 +638,59 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   if (m->device != c->device) return fail(c, SK_EINVAL, "matrix belongs to another device");
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
-  auto run = [&](int v) {
-    if (v == 2) {
-      dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-      hipLaunchKernelGGL(k_spmv_wide<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
-                         (long)m->rows, (long)m->cols, d_in, d_out);
+  // Variants 3, 4 (SK_MAT_FP): the steep rays gather from x^T, a transposed copy of the input
+  // images in the context scratch (one transpose per call, ~10 us at 2048^2).
+  const bool fpT = m->kind == SK_MAT_FP && m->steep != nullptr;
+  float* xT = nullptr;
+  auto need_xT = [&]() -> int {
+    if (xT) return SK_OK;
+    void* ws = nullptr;
+    int rc = scratch(c, sizeof(float) * (size_t)batch * m->cols, s, &ws);
+    if (rc) return rc;
+    xT = (float*)ws;
+    return launch_transpose(c, d_in, xT, m->N, batch, s);
+  };
+  const double invN = m->N > 0 ? 1.0 / (double)m->N : 0.0;
+  auto run = [&](int v) -> int {
+    dim3 grid((unsigned)((m->rows + 3) / 4), batch);
+    if (v == 3 || v == 4) {
+      int rc = need_xT();
+      if (rc) return rc;
+      if (v == 3)
+        hipLaunchKernelGGL((k_spmv<4, true>), grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                           (long)m->rows, (long)m->cols, d_in, d_out, xT, m->steep, m->R, m->N, invN);
+      else
+        hipLaunchKernelGGL((k_spmv_wide<4, true>), grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                           (long)m->rows, (long)m->cols, d_in, d_out, xT, m->steep, m->R, m->N, invN);
+    } else if (v == 2) {
+      hipLaunchKernelGGL((k_spmv_wide<4, false>), grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                         (long)m->rows, (long)m->cols, d_in, d_out, nullptr, nullptr, 1, 0, 0.0);
     } else if (v == 1) {
-      dim3 grid((unsigned)((m->rows + 255) / 256), batch);
-      hipLaunchKernelGGL(k_spmv_lockstep, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+      dim3 g1((unsigned)((m->rows + 255) / 256), batch);
+      hipLaunchKernelGGL(k_spmv_lockstep, g1, dim3(256), 0, s, m->rowptr, m->col, m->val,
                          (long)m->rows, (long)m->cols, d_in, d_out);
     } else {
-      dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-      hipLaunchKernelGGL(k_spmv<4>, grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
-                         (long)m->rows, (long)m->cols, d_in, d_out);
+      hipLaunchKernelGGL((k_spmv<4, false>), grid, dim3(256), 0, s, m->rowptr, m->col, m->val,
+                         (long)m->rows, (long)m->cols, d_in, d_out, nullptr, nullptr, 1, 0, 0.0);
     }
+    return SK_OK;
   };
   if (m->spmv_variant < 0) {
-    // First use: time the wave-per-row kernels once on this matrix and keep the faster.
+    // First use: time the wave-per-row kernels once on this matrix and keep the fastest.
     // (The lockstep kernel, variant 1, measured 6-9x slower on both matrices at 2048^2:
     // each of its load instructions touches 64 rows' cache lines.  It is selectable only.)
-    constexpr int NV = 2;
-    const int cand[NV] = {0, 2};
-    hipEvent_t e[NV + 1];
-    for (auto& ev : e) SK_HIP(c, hipEventCreate(&ev));
+    const int NV = fpT ? 4 : 2;
+    const int cand[4] = {0, 2, 3, 4};
+    hipEvent_t e[5];
+    for (int v = 0; v <= NV; ++v) SK_HIP(c, hipEventCreate(&e[v]));
+    if (fpT) {
+      int rc = need_xT();  // outside the timed region of variants 3, 4 (as for every later call)
+      if (rc) return rc;
+    }
     SK_HIP(c, hipEventRecord(e[0], s));
     for (int v = 0; v < NV; ++v) {
-      run(cand[v]);
+      int rc = run(cand[v]);
+      if (rc) return rc;
       SK_HIP(c, hipEventRecord(e[v + 1], s));
     }
     SK_HIP(c, hipEventSynchronize(e[NV]));
@@ -605,14 +705,18 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
         best = cand[v];
       }
     }
-    for (auto& ev : e) (void)hipEventDestroy(ev);
+    for (int v = 0; v <= NV; ++v) (void)hipEventDestroy(e[v]);
     const_cast<sk_mat*>(m)->spmv_variant = best;
     // The kernels sum a row in different orders; leave the chosen one's result, so that
     // every call on this matrix returns the same bits.
-    if (best != cand[NV - 1]) run(best);
+    if (best != cand[NV - 1]) {
+      int rc = run(best);
+      if (rc) return rc;
+    }
     return launched(c, "SpMV kernel");
   }
-  run(m->spmv_variant);
+  int rc = run(m->spmv_variant);
+  if (rc) return rc;
   return launched(c, "SpMV kernel");
 }
 

@@ -212,12 +212,14 @@ class LearnedSIRT:
         """Returns (images, fell_back [B] bool)."""
         B = b.shape[0]
         x = torch.zeros((B, self.geom.N, self.geom.N), device=b.device)
-        xp = torch.zeros_like(x)
-        for _ in range(self.stages):
+        xp = None
+        for k in range(self.stages):
             xs = self.sirt_step(x, b)
+            # both paths start from zero, so plain SIRT's first step is the first stage's (the
+            # kernels are deterministic: the same bits)
+            xp = xs.clone() if k == 0 else self.sirt_step(xp, b)
             xc = self.net(xs)
             bad = ~torch.isfinite(xc).reshape(B, -1).all(dim=1)
             x = torch.where(bad.view(B, 1, 1), xs, xc)
-            xp = self.sirt_step(xp, b)
         worse = residual(self.geom, x, b) > residual(self.geom, xp, b)
         return torch.where(worse.view(B, 1, 1).to(x.device), xp, x), worse

@@ -312,7 +312,9 @@ class Matrix:
         return rp, c[: self.nnz], v[: self.nnz]
 
     def set_spmv_variant(self, variant: int) -> None:
-        """0: wave per row, 1: lane per row (lockstep), -1: time both on next use."""
+        """0: wave per row, 1: lane per row (lockstep), 2: wave per row with 16-byte streams,
+        3 / 4: as 0 / 2 with the steep rays' gathers from x^T (A only); -1: time the
+        wave-per-row kernels on next use and keep the fastest."""
         nat.check(nat.lib().sk_mat_set_spmv_variant(self.handle, int(variant)), self.geom.ctx.handle)
 
     def spmv(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -60,6 +60,25 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
     geo.drop_sparse()
 
 
+@pytest.mark.parametrize("name,N,ang,R,rz,dr", GEOMS, ids=[g[0] for g in GEOMS])
+def test_spmv_transposed_gathers_bitwise(ctx, name, N, ang, R, rz, dr):
+    """Variants 3 and 4 of A (steep rays gather from x^T) sum the same numbers in the same
+    order as variants 0 and 2: bitwise equal, batched too; B rejects them."""
+    rng = np.random.default_rng(13)
+    x = _dev(rng.random((3, N * N)).astype(np.float32))
+    geo = ctx.geometry(ang, rz, dr, R, N)
+    A = geo.sparse("fp")
+    out = {}
+    for variant in (0, 2, 3, 4):
+        A.set_spmv_variant(variant)
+        out[variant] = A.spmv(x).cpu().numpy()
+    np.testing.assert_array_equal(out[3], out[0])
+    np.testing.assert_array_equal(out[4], out[2])
+    with pytest.raises(ValueError):
+        geo.sparse("bp").set_spmv_variant(3)
+    geo.drop_sparse()
+
+
 def _sorted_entries(rows, cols, vals):
     order = np.lexsort((cols, rows))
     return rows[order], cols[order], vals[order]
