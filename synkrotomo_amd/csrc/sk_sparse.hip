@@ -26,6 +26,7 @@
 // wavefront-cooperative: one wave per row, lanes over the row's steps (A: 64 image rows at
 // a time; B: 64 lines at a time); ballots and mbcnt place each lane's entries, so the
 // entries come out in the sequential order above and the writes are contiguous.
+#include <algorithm>
 #include <cstdlib>
 #include <vector>
 
@@ -35,12 +36,21 @@ struct sk_mat {
   int kind = 0;
   int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
                           // 16-byte streams; 3, 4: as 0, 2 with the gathers of steep rays
-                          // from x^T (SK_MAT_FP only); -1: not yet timed
+                          // from x^T (SK_MAT_FP only); 7: SELL-64 copy; -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   // SK_MAT_FP: per original angle, 1 if its line is steep (its rays step through image rows:
   // in x^T their pixels are contiguous), and the geometry's R and N
   unsigned char* steep = nullptr;
   int R = 0, N = 0;
+  // SELL-64 copy (variant 7, built on first use): rows in slices of 64; slice k holds
+  // L_k = its longest row's length columns of 64 entries, entry j of row 64k + l at
+  // sell_off[k] + 64j + l (zero padding past each row's end), so a wave reads both streams
+  // coalesced and its 64 gathers belong to 64 neighbouring rows.
+  int64_t* sell_off = nullptr;  // [slices + 1]
+  int32_t* sell_col = nullptr;
+  float* sell_val = nullptr;
+  int32_t* sell_len = nullptr;  // [rows] row lengths
+  bool sell_failed = false;     // no memory for the copy: variant 7 unavailable
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
   float* val = nullptr;       // [nnz]
@@ -392,6 +402,77 @@ __device__ __forceinline__ float spmv_row_wide(const int32_t* __restrict__ col, 
   return acc;
 }
 
+// ------------------------------------------------------------------ SELL-64 (variant 7)
+// Per slice: 64 * (longest row) entries; per row: its length.
+__global__ __launch_bounds__(256) void k_sell_sizes(const int64_t* __restrict__ rowptr, long rows,
+                                                    int32_t* __restrict__ slice_entries,
+                                                    int32_t* __restrict__ len) {
+  const long row = (long)blockIdx.x * 256 + threadIdx.x;  // 4 slices per block
+  const int n = row < rows ? (int)(rowptr[row + 1] - rowptr[row]) : 0;
+  if (row < rows) len[row] = n;
+  int m = n;
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0 && row < rows) slice_entries[row >> 6] = 64 * m;
+}
+
+// One wave per slice: lane l copies row 64k + l's entries into columns j = 0 .. L_k - 1.
+__global__ __launch_bounds__(256) void k_sell_fill(const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col,
+                                                   const float* __restrict__ val, long rows,
+                                                   long slices, const int64_t* __restrict__ off,
+                                                   int32_t* __restrict__ scol, float* __restrict__ sval) {
+  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slice >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const long row = slice * 64 + lane;
+  const long beg = row < rows ? rowptr[row] : 0;
+  const long n = row < rows ? rowptr[row + 1] - beg : 0;
+  const long L = (off[slice + 1] - off[slice]) >> 6;
+  for (long j = 0; j < L; ++j) {
+    const bool in = j < n;
+    scol[off[slice] + 64 * j + lane] = in ? col[beg + j] : 0;
+    sval[off[slice] + 64 * j + lane] = in ? val[beg + j] : 0.0f;
+  }
+}
+
+// One wave per slice, one lane per row, entries in CSR order (the lockstep kernel's order and
+// operations: bitwise equal to variant 1); entries past a row's end are skipped.
+template <int U>
+__global__ __launch_bounds__(256) void k_spmv_sell(const int64_t* __restrict__ off,
+                                                   const int32_t* __restrict__ scol,
+                                                   const float* __restrict__ sval,
+                                                   const int32_t* __restrict__ len, long rows,
+                                                   long slices, long ncols, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slice >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const long row = slice * 64 + lane;
+  const int n = row < rows ? len[row] : 0;
+  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const long base = off[slice];
+  const int L = (int)((off[slice + 1] - base) >> 6);
+  const int32_t* c = scol + base + lane;
+  const float* v = sval + base + lane;
+  float acc = 0.0f;
+  int j = 0;
+  for (; j + U <= L; j += U) {
+    int cc[U];
+    float vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cc[u] = __builtin_nontemporal_load(c + 64 * (j + u));
+      vv[u] = __builtin_nontemporal_load(v + 64 * (j + u));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j + u < n) acc = fmaf(vv[u], xb[cc[u]], acc);
+  }
+  for (; j < L; ++j)
+    if (j < n) acc = fmaf(v[64 * j], xb[c[64 * j]], acc);
+  if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
 // counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
 static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr, int64_t* nnz,
                        hipStream_t s) {
@@ -488,7 +569,55 @@ static void mat_free(sk_mat* m) {
   if (m->col) (void)hipFree(m->col);
   if (m->val) (void)hipFree(m->val);
   if (m->steep) (void)hipFree(m->steep);
+  if (m->sell_off) (void)hipFree(m->sell_off);
+  if (m->sell_col) (void)hipFree(m->sell_col);
+  if (m->sell_val) (void)hipFree(m->sell_val);
+  if (m->sell_len) (void)hipFree(m->sell_len);
   delete m;
+}
+
+// The SELL-64 copy of a matrix (variant 7).  With `required` false it is built only if it
+// takes at most half the free device memory (the autotuner's candidate); otherwise it is
+// marked unavailable.  Synchronizes `s` (sizes).
+static int build_sell(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
+  if (m->sell_col || (m->sell_failed && !required)) return SK_OK;
+  const long slices = (m->rows + 63) / 64;
+  int32_t* sizes = nullptr;
+  SK_HIP(c, hipMalloc(&sizes, sizeof(int32_t) * (size_t)slices));
+  hipError_t e = hipSuccess;
+  if (!m->sell_len) e = hipMalloc(&m->sell_len, sizeof(int32_t) * (size_t)m->rows);
+  if (e == hipSuccess && !m->sell_off) e = hipMalloc(&m->sell_off, sizeof(int64_t) * (size_t)(slices + 1));
+  if (e != hipSuccess) {
+    (void)hipFree(sizes);
+    return hip_fail(c, e, "SELL index allocation");
+  }
+  hipLaunchKernelGGL(k_sell_sizes, dim3((unsigned)((m->rows + 255) / 256)), dim3(256), 0, s, m->rowptr,
+                     (long)m->rows, sizes, m->sell_len);
+  int rc = launched(c, "SELL size kernel");
+  int64_t total = 0;
+  if (!rc) rc = scan_counts(c, sizes, slices, m->sell_off, &total, s);
+  (void)hipFree(sizes);
+  if (rc) return rc;
+  size_t free_b = 0, total_b = 0;
+  SK_HIP(c, hipMemGetInfo(&free_b, &total_b));
+  const size_t need = (size_t)total * (sizeof(int32_t) + sizeof(float));
+  if (!required && need > free_b / 2) {
+    m->sell_failed = true;
+    return SK_OK;
+  }
+  e = hipMalloc(&m->sell_col, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1));
+  if (e == hipSuccess) e = hipMalloc(&m->sell_val, sizeof(float) * (size_t)std::max<int64_t>(total, 1));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (m->sell_col) (void)hipFree(m->sell_col);
+    m->sell_col = nullptr;
+    m->sell_val = nullptr;
+    m->sell_failed = true;
+    return required ? hip_fail(c, e, "SELL copy allocation") : SK_OK;
+  }
+  hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((slices + 3) / 4)), dim3(256), 0, s, m->rowptr, m->col, m->val,
+                     (long)m->rows, slices, m->sell_off, m->sell_col, m->sell_val);
+  return launched(c, "SELL fill kernel");
 }
 
 extern "C" {
@@ -589,8 +718,8 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 4) return SK_EINVAL;
-  if (variant >= 3 && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 7 || variant == 5 || variant == 6) return SK_EINVAL;
+  if ((variant == 3 || variant == 4) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -653,7 +782,13 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   const double invN = m->N > 0 ? 1.0 / (double)m->N : 0.0;
   auto run = [&](int v) -> int {
     dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-    if (v == 3 || v == 4) {
+    if (v == 7) {
+      int rc = build_sell(c, const_cast<sk_mat*>(m), true, s);
+      if (rc) return rc;
+      const long slices = (m->rows + 63) / 64;
+      hipLaunchKernelGGL(k_spmv_sell<4>, dim3((unsigned)((slices + 3) / 4), batch), dim3(256), 0, s, m->sell_off,
+                         m->sell_col, m->sell_val, m->sell_len, (long)m->rows, slices, (long)m->cols, d_in, d_out);
+    } else if (v == 3 || v == 4) {
       int rc = need_xT();
       if (rc) return rc;
       if (v == 3)
@@ -679,9 +814,19 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     // First use: time the wave-per-row kernels once on this matrix and keep the fastest.
     // (The lockstep kernel, variant 1, measured 6-9x slower on both matrices at 2048^2:
     // each of its load instructions touches 64 rows' cache lines.  It is selectable only.)
-    const int NV = fpT ? 4 : 2;
-    const int cand[4] = {0, 2, 3, 4};
-    hipEvent_t e[5];
+    // SELL-64 (7) is a candidate when its copy fits in half the free memory (built here,
+    // outside the timed region).
+    {
+      int rc = build_sell(c, const_cast<sk_mat*>(m), false, s);
+      if (rc) return rc;
+    }
+    const bool sell = m->sell_col != nullptr;
+    // (8 loads in flight per lane instead of 4 measured no faster: 45.8 vs 43.9 ms for A,
+    // 63.5 vs 62.6 ms for B at 2048^2.)
+    int cand[5] = {0, 2, 3, 4, 7};
+    int NV = fpT ? 4 : 2;
+    if (sell) cand[NV++] = 7;
+    hipEvent_t e[6];
     for (int v = 0; v <= NV; ++v) SK_HIP(c, hipEventCreate(&e[v]));
     if (fpT) {
       int rc = need_xT();  // outside the timed region of variants 3, 4 (as for every later call)

@@ -75,6 +75,7 @@ def train(args) -> int:
     sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=args.lr, total_steps=steps, pct_start=0.05)
     t0 = time.perf_counter()
     log = []
+    skipped = 0
     for step in range(steps):
         if step % (args.phantoms // args.batch) == 0:
             rng = np.random.default_rng(TRAIN_SEED)  # each epoch revisits the same phantoms
@@ -90,7 +91,10 @@ def train(args) -> int:
             from .autograd import project
             cc = clean.reshape(b.shape[0], -1)
             d = project(x, geo).reshape(b.shape[0], -1) - cc
-            loss = loss + args.proj_weight * torch.mean(d.square().sum(1) / cc.square().sum(1))
+            # a phantom whose ellipses all clip to zero has no projection: no term for it
+            den = cc.square().sum(1)
+            loss = loss + args.proj_weight * torch.mean(torch.where(den > 0, d.square().sum(1) / den.clamp_min(1e-30),
+                                                                    torch.zeros_like(den)))
         if args.stage_weight > 0:
             for xs in stages[:-1]:
                 loss = loss + args.stage_weight * torch.mean((xs - x_true) ** 2)
@@ -109,12 +113,23 @@ def train(args) -> int:
                 excess = torch.relu(r2.clamp_min(1e-20).sqrt() - 1.1 * floor)
                 loss = loss + args.res_excess_weight * torch.mean(excess.square())
         opt.zero_grad(set_to_none=True)
+        if not torch.isfinite(loss):
+            # a non-finite loss would poison Adam's moments: skip the batch, keep the schedule
+            skipped += 1
+            sched.step()
+            continue
         loss.backward()
+        grads_ok = torch.stack([torch.isfinite(q.grad).all() for q in net.parameters() if q.grad is not None]).all()
+        if not bool(grads_ok):
+            skipped += 1
+            sched.step()
+            continue
         opt.step()
         sched.step()
         if step % 100 == 0 or step == steps - 1:
             torch.cuda.synchronize()
-            rec = {"step": step, "loss": float(loss.detach()), "elapsed_s": round(time.perf_counter() - t0, 1)}
+            rec = {"step": step, "loss": float(loss.detach()), "skipped": skipped,
+                   "elapsed_s": round(time.perf_counter() - t0, 1)}
             log.append(rec)
             print(json.dumps(rec), flush=True)
     torch.cuda.synchronize()
