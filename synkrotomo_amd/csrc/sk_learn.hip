@@ -7,10 +7,11 @@
 // accumulation are FP32; weights are BF16 in [Cout][ky][kx][Cin] layout.
 //
 //   k_conv32      32 -> 32 (the FLOP-heavy layers) as an implicit GEMM on gfx950 MFMA
-//                 (v_mfma_f32_16x16x32_bf16): M = 16 output pixels of one row, N = 16 output
-//                 channels, K = 32 input channels of one tap; 9 taps accumulate.  A block
-//                 computes a 16x16-pixel tile x 32 channels; the 18x18x32 input patch and the
-//                 32x9x32 weights are staged in LDS.  The same kernel computes the data
+//                 (v_mfma_f32_16x16x32_bf16): M = 16 output channels, N = 16 output pixels
+//                 of one row, K = 32 input channels of one tap; 9 taps accumulate.  Persistent
+//                 blocks compute 16x16-pixel x 32-channel tiles from an LDS-staged 18x18x32
+//                 input patch (the next tile's patch loaded during the MFMAs) and the 32x9x32
+//                 weights, staged once per block.  The same kernel computes the data
 //                 gradient (weights flipped and transposed on the host side), with the ReLU
 //                 derivative of the layer's output fused into the staging of its input.
 //   k_conv_in/out 1 -> 32 and 32 -> 1 layers (288 MACs per pixel) on VALU, fused bias, ReLU
@@ -27,6 +28,7 @@
 namespace sk {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CT = 16;    // conv tile side (pixels)
@@ -34,8 +36,16 @@ constexpr int CP = 40;    // LDS channel stride (bf16 elements): 80 B, keeps 16 
 constexpr int NCH = 32;   // hidden channels
 
 // ---------------------------------------------------------------------- 32 -> 32, MFMA
+// Persistent blocks: each stages the 32x9x32 weights in LDS once, then walks output tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ...  A tile's 18x18x32 input patch is loaded into
+// registers while the previous tile's MFMAs run, then stored to LDS.  The MFMA's M is the 16
+// output channels of a channel block (A = weights), N the 16 pixels of a row (B = input), so
+// each lane's accumulator holds 4 consecutive channels of one pixel: 8-byte stores.
+constexpr int PATCH_Q = (CT + 2) * (CT + 2) * 4;      // 16-byte quads of a patch
+constexpr int PATCH_PER_T = (PATCH_Q + 255) / 256;    // per thread
+
 template <bool MASK_IN, bool BIAS_RELU>
-__global__ __launch_bounds__(256) void k_conv32(int H, int W, const __bf16* __restrict__ x,
+__global__ __launch_bounds__(256) void k_conv32(int H, int W, long ntiles, const __bf16* __restrict__ x,
                                                 const __bf16* __restrict__ mask,
                                                 const __bf16* __restrict__ w,
                                                 const float* __restrict__ bias,
@@ -43,63 +53,90 @@ __global__ __launch_bounds__(256) void k_conv32(int H, int W, const __bf16* __re
   __shared__ __attribute__((aligned(16))) __bf16 sx[(CT + 2) * (CT + 2) * CP];
   __shared__ __attribute__((aligned(16))) __bf16 sw[NCH * 9 * CP];
   const int tiles_x = (W + CT - 1) / CT;
-  const int tx0 = (blockIdx.x % tiles_x) * CT, ty0 = (blockIdx.x / tiles_x) * CT;
-  const size_t img = (size_t)blockIdx.y * H * W;
+  const long tiles_img = (long)tiles_x * ((H + CT - 1) / CT);
   for (int c = threadIdx.x; c < NCH * 9 * 4; c += 256) {
     const int row = c >> 2, q = c & 3;  // row = co*9 + tap
     *(bf16x8*)(sw + row * CP + q * 8) = *(const bf16x8*)(w + row * NCH + q * 8);
   }
-  for (int c = threadIdx.x; c < (CT + 2) * (CT + 2) * 4; c += 256) {
-    const int pix = c >> 2, q = c & 3;
-    const int py = pix / (CT + 2), px = pix - py * (CT + 2);
-    const int gy = ty0 + py - 1, gx = tx0 + px - 1;
-    bf16x8 v;
-    for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
-    if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
-      const size_t off = (img + (size_t)gy * W + gx) * NCH + q * 8;
-      v = *(const bf16x8*)(x + off);
-      if constexpr (MASK_IN) {
-        const bf16x8 m = *(const bf16x8*)(mask + off);
-        for (int i = 0; i < 8; ++i)
-          if (!((float)m[i] > 0.0f)) v[i] = (__bf16)0.0f;
+  bf16x8 pf[PATCH_PER_T];
+  auto load_patch = [&](long t) {
+    const long b = t / tiles_img;
+    const int r = (int)(t - b * tiles_img);
+    const int tx0 = (r % tiles_x) * CT, ty0 = (r / tiles_x) * CT;
+    const size_t img = (size_t)b * H * W;
+#pragma unroll
+    for (int k = 0; k < PATCH_PER_T; ++k) {
+      const int c = threadIdx.x + 256 * k;
+      bf16x8 v;
+      for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
+      const int pix = c >> 2, q = c & 3;
+      const int py = pix / (CT + 2), px = pix - py * (CT + 2);
+      const int gy = ty0 + py - 1, gx = tx0 + px - 1;
+      if (c < PATCH_Q && gy >= 0 && gy < H && gx >= 0 && gx < W) {
+        const size_t off = (img + (size_t)gy * W + gx) * NCH + q * 8;
+        v = *(const bf16x8*)(x + off);
+        if constexpr (MASK_IN) {
+          const bf16x8 m = *(const bf16x8*)(mask + off);
+          for (int i = 0; i < 8; ++i)
+            if (!((float)m[i] > 0.0f)) v[i] = (__bf16)0.0f;
+        }
       }
+      pf[k] = v;
     }
-    *(bf16x8*)(sx + pix * CP + q * 8) = v;
-  }
-  __syncthreads();
+  };
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  f32x4 acc[4][2];
-  for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  long t = blockIdx.x;
+  if (t < ntiles) load_patch(t);
+  for (; t < ntiles; t += gridDim.x) {  // uniform over the block: every thread reaches each barrier
+    __syncthreads();  // the previous tile's LDS reads (and, first, the weight stores) are done
 #pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int ky = t / 3, kx = t - 3 * (t / 3);
-    const bf16x8 b0 = *(const bf16x8*)(sw + ((0 * 16 + m) * 9 + t) * CP + 8 * kq);
-    const bf16x8 b1 = *(const bf16x8*)(sw + ((1 * 16 + m) * 9 + t) * CP + 8 * kq);
+    for (int k = 0; k < PATCH_PER_T; ++k) {
+      const int c = threadIdx.x + 256 * k;
+      if (c < PATCH_Q) *(bf16x8*)(sx + (c >> 2) * CP + (c & 3) * 8) = pf[k];
+    }
+    __syncthreads();
+    const long tn = t + gridDim.x;
+    if (tn < ntiles) load_patch(tn);  // in flight during this tile's MFMAs
+    const long b = t / tiles_img;
+    const int r = (int)(t - b * tiles_img);
+    const int tx0 = (r % tiles_x) * CT, ty0 = (r / tiles_x) * CT;
+    const size_t img = (size_t)b * H * W;
+    f32x4 acc[4][2];
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 2; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) {
+      const int ky = tp / 3, kx = tp - 3 * (tp / 3);
+      // A: weights of output channels nb*16 + m, input channels 8kq..8kq+7 of tap tp
+      const bf16x8 w0 = *(const bf16x8*)(sw + ((0 * 16 + m) * 9 + tp) * CP + 8 * kq);
+      const bf16x8 w1 = *(const bf16x8*)(sw + ((1 * 16 + m) * 9 + tp) * CP + 8 * kq);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const int oy = wv * 4 + mb;
+        // B: input channels 8kq..8kq+7 of pixel m of output row oy, shifted by the tap
+        const bf16x8 a = *(const bf16x8*)(sx + ((oy + ky) * (CT + 2) + m + kx) * CP + 8 * kq);
+        acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, a, acc[mb][0], 0, 0, 0);
+        acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, a, acc[mb][1], 0, 0, 0);
+      }
+    }
+    // C: lane holds output channels nb*16 + 4kq + i (i < 4) of pixel m of row oy.
+    const int ox = tx0 + m;
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      const int oy = wv * 4 + mb;
-      const bf16x8 a = *(const bf16x8*)(sx + ((oy + ky) * (CT + 2) + m + kx) * CP + 8 * kq);
-      acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[mb][0], 0, 0, 0);
-      acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[mb][1], 0, 0, 0);
-    }
-  }
-  // C layout: lane holds rows (pixels) 4*kq + i, column (channel) m of each 16x16 block.
+      const int oy = ty0 + wv * 4 + mb;
+      if (oy >= H || ox >= W) continue;
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) {
-    const int oy = ty0 + wv * 4 + mb;
-    if (oy >= H) continue;
+      for (int nb = 0; nb < 2; ++nb) {
+        const int co = nb * 16 + 4 * kq;
+        bf16x4 o;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      const int co = nb * 16 + m;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ox = tx0 + 4 * kq + i;
-        if (ox >= W) continue;
-        float v = acc[mb][nb][i];
-        if constexpr (BIAS_RELU) v = fmaxf(v + bias[co], 0.0f);
-        y[(img + (size_t)oy * W + ox) * NCH + co] = (__bf16)v;
+        for (int i = 0; i < 4; ++i) {
+          float v = acc[mb][nb][i];
+          if constexpr (BIAS_RELU) v = fmaxf(v + bias[co + i], 0.0f);
+          o[i] = (__bf16)v;
+        }
+        *(bf16x4*)(y + (img + (size_t)oy * W + ox) * NCH + co) = o;
       }
     }
   }
@@ -356,18 +393,28 @@ int sk_conv32(sk_ctx* ctx, int B, int H, int W, const void* x, const void* mask,
   if (B < 1 || B > 65535 || H < 1 || W < 1 || !x || !w || !y)
     return fail(&ctx->c, SK_EINVAL, "conv32: bad arguments");
   const bool relu = bias != nullptr;
-  dim3 grid(((W + CT - 1) / CT) * ((H + CT - 1) / CT), B);
+  const long ntiles = (long)((W + CT - 1) / CT) * ((H + CT - 1) / CT) * B;
+  // persistent grid: the resident blocks of the chip (queried once per process and device)
+  static int per_cu[64] = {0}, cus[64] = {0};
+  const int dev = ctx->c.device < 64 ? ctx->c.device : 63;
+  if (!per_cu[dev]) {
+    SK_HIP(&ctx->c, hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, ctx->c.device));
+    SK_HIP(&ctx->c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[dev], k_conv32<true, true>, 256, 0));
+    per_cu[dev] = std::max(1, per_cu[dev]);
+  }
+  const long resident = (long)cus[dev] * per_cu[dev];
+  dim3 grid((unsigned)std::min(ntiles, resident));
   hipStream_t s = (hipStream_t)stream;
   const __bf16 *X = (const __bf16*)x, *Mk = (const __bf16*)mask, *Wt = (const __bf16*)w;
   __bf16* Y = (__bf16*)y;
   if (mask && relu)
-    hipLaunchKernelGGL((k_conv32<true, true>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+    hipLaunchKernelGGL((k_conv32<true, true>), grid, dim3(256), 0, s, H, W, ntiles, X, Mk, Wt, bias, Y);
   else if (mask)
-    hipLaunchKernelGGL((k_conv32<true, false>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+    hipLaunchKernelGGL((k_conv32<true, false>), grid, dim3(256), 0, s, H, W, ntiles, X, Mk, Wt, bias, Y);
   else if (relu)
-    hipLaunchKernelGGL((k_conv32<false, true>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+    hipLaunchKernelGGL((k_conv32<false, true>), grid, dim3(256), 0, s, H, W, ntiles, X, Mk, Wt, bias, Y);
   else
-    hipLaunchKernelGGL((k_conv32<false, false>), grid, dim3(256), 0, s, H, W, X, Mk, Wt, bias, Y);
+    hipLaunchKernelGGL((k_conv32<false, false>), grid, dim3(256), 0, s, H, W, ntiles, X, Mk, Wt, bias, Y);
   return launched(&ctx->c, "conv32 kernel");
 }
 
