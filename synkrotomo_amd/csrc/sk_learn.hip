@@ -14,8 +14,11 @@
 //                 weights, staged once per block.  The same kernel computes the data
 //                 gradient (weights flipped and transposed on the host side), with the ReLU
 //                 derivative of the layer's output fused into the staging of its input.
-//   k_conv_in/out 1 -> 32 and 32 -> 1 layers (288 MACs per pixel) on VALU, fused bias, ReLU
-//                 and residual; their data gradients likewise.
+//   k_conv_in     1 -> 32 (288 MACs per pixel) on VALU with fused bias and ReLU, in FP32 on the
+//                 FP32 image; the data gradients of both thin layers likewise.
+//   k_conv_out_mfma  32 -> 1: per 16x16 tile, the 9 tap products of every halo pixel in one
+//                 MFMA per 16 pixels, then a 9-term shifted sum per output, fused bias and
+//                 residual.
 //   k_wgrad       weight and bias gradients: per-block partial sums over pixel tiles
 //                 (deterministic), then k_reduce_partials.
 //   k_phantom     random-ellipse phantoms, evaluated in FP64 exactly as the numpy
@@ -176,35 +179,53 @@ __global__ __launch_bounds__(256) void k_conv_in(int H, int W, long npix, const 
   }
 }
 
-// ---------------------------------------------------------------------- 32 -> 1, VALU
-// y[p] = res[p] + b + sum_{t,ci} w[t][ci] h[p + off_t][ci]; w: [9][32] bf16
-__global__ __launch_bounds__(256) void k_conv_out(int H, int W, long npix, const __bf16* __restrict__ h,
-                                                  const __bf16* __restrict__ w,
-                                                  const float* __restrict__ bias,
-                                                  const float* __restrict__ res,
-                                                  float* __restrict__ y) {
-  __shared__ float sw[9 * NCH];
-  for (int i = threadIdx.x; i < 9 * NCH; i += 256) sw[i] = (float)w[i];
-  __syncthreads();
-  const long p = (long)blockIdx.x * 256 + threadIdx.x;
-  if (p >= npix) return;
-  const long b = p / ((long)H * W);
-  const int r = (int)(p - b * H * W);
-  const int py = r / W, px = r - py * W;
-  float s = 0.0f;
-  for (int t = 0; t < 9; ++t) {
-    const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-    if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
-    const __bf16* hp = h + ((b * H + yy) * W + xx) * NCH;
-    for (int q = 0; q < 4; ++q) {
-      const bf16x8 v = *(const bf16x8*)(hp + q * 8);
-      for (int i = 0; i < 8; ++i) s = fmaf(sw[t * NCH + q * 8 + i], (float)v[i], s);
-    }
+// 32 -> 1 on MFMA: for a 16x16 output tile, the 18x18 input pixels of its halo each get
+// U[q][t] = sum_ci h[q][ci] w[t][ci] for the 9 taps t in one GEMM ([pixels x 32] x [32 x 16],
+// taps 9..15 zero: one v_mfma_f32_16x16x32_bf16 per 16 pixels), kept in LDS; each output
+// pixel then sums its 9 shifted taps: y[p] = res[p] + (sum_t U[p + off_t][t] + bias).
+// Each activation is read once per tile (the halo adds 27%).
+constexpr int CO_SLOTS = 21 * 16;  // 18*18 = 324 halo pixels in groups of 16
+
+__global__ __launch_bounds__(256) void k_conv_out_mfma(int H, int W, const __bf16* __restrict__ h,
+                                                       const __bf16* __restrict__ w,
+                                                       const float* __restrict__ bias,
+                                                       const float* __restrict__ res,
+                                                       float* __restrict__ y) {
+  __shared__ float U[CO_SLOTS * 9];
+  const int tiles_x = (W + CT - 1) / CT;
+  const int tx0 = (blockIdx.x % tiles_x) * CT, ty0 = (blockIdx.x / tiles_x) * CT;
+  const size_t img = (size_t)blockIdx.y * H * W;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  // B: w[tap m][8kq .. 8kq+7] for taps 0..8, zero columns for m >= 9
+  bf16x8 wb;
+  for (int i = 0; i < 8; ++i) wb[i] = (__bf16)0.0f;
+  if (m < 9) wb = *(const bf16x8*)(w + m * NCH + 8 * kq);
+  for (int g = wv; g < CO_SLOTS / 16; g += 4) {
+    const int slot = g * 16 + m;  // A row m: halo pixel `slot`
+    const int hy = slot / (CT + 2), hx = slot - hy * (CT + 2);
+    const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
+    bf16x8 a;
+    for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
+    if (slot < (CT + 2) * (CT + 2) && gy >= 0 && gy < H && gx >= 0 && gx < W)
+      a = *(const bf16x8*)(h + (img + (size_t)gy * W + gx) * NCH + 8 * kq);
+    const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    // C: lane holds U[pixel g*16 + 4kq + i][tap m]
+    if (m < 9)
+      for (int i = 0; i < 4; ++i) U[(g * 16 + 4 * kq + i) * 9 + m] = c[i];
   }
-  y[p] = res[p] + (s + bias[0]);
+  __syncthreads();
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  const int oy = ty0 + ty, ox = tx0 + tx;
+  if (oy >= H || ox >= W) return;
+  float sum = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) sum += U[((ty + t / 3) * (CT + 2) + tx + t % 3) * 9 + t];
+  const size_t p = img + (size_t)oy * W + ox;
+  y[p] = res[p] + (sum + bias[0]);
 }
 
-// d h[q][ci] = sum_t w[t][ci] g[q - off_t]   (data gradient of k_conv_out)
+// d h[q][ci] = sum_t w[t][ci] g[q - off_t]   (data gradient of the 32 -> 1 layer)
 __global__ __launch_bounds__(256) void k_conv_out_dgrad(int H, int W, long npix, const float* __restrict__ g,
                                                         const __bf16* __restrict__ w,
                                                         __bf16* __restrict__ dh) {
@@ -435,9 +456,10 @@ int sk_conv_out(sk_ctx* ctx, int B, int H, int W, const void* h, const void* w, 
   if (rc) return rc;
   if (B < 1 || H < 1 || W < 1 || !h || !w || !bias || !res || !y)
     return fail(&ctx->c, SK_EINVAL, "conv_out: bad arguments");
-  const long npix = (long)B * H * W;
-  hipLaunchKernelGGL(k_conv_out, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream, H, W,
-                     npix, (const __bf16*)h, (const __bf16*)w, bias, res, y);
+  if (B > 65535) return fail(&ctx->c, SK_EINVAL, "conv_out: batch must be <= 65535");
+  dim3 grid(((W + CT - 1) / CT) * ((H + CT - 1) / CT), B);
+  hipLaunchKernelGGL(k_conv_out_mfma, grid, dim3(256), 0, (hipStream_t)stream, H, W, (const __bf16*)h,
+                     (const __bf16*)w, bias, res, y);
   return launched(&ctx->c, "conv_out kernel");
 }
 
