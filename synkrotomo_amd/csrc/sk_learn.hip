@@ -24,6 +24,7 @@
 //   k_phantom     random-ellipse phantoms, evaluated in FP64 exactly as the numpy
 //                 generator (synkrotomo_amd/phantom.py) does.
 #include <algorithm>
+#include <cstdint>
 #include <type_traits>
 
 #include "sk_common.h"
@@ -147,35 +148,52 @@ __global__ __launch_bounds__(256) void k_conv32(int H, int W, long ntiles, const
 
 // ---------------------------------------------------------------------- 1 -> 32, VALU
 // h[p][co] = relu(b[co] + sum_t w[co][t] x[p + off_t]); w: [32][9] bf16
-__global__ __launch_bounds__(256) void k_conv_in(int H, int W, long npix, const float* __restrict__ x,
-                                                 const __bf16* __restrict__ w,
+// Grid (pixel blocks of one image, images).  The weights are read as BF16 pairs through
+// readfirstlane (wave-uniform values: scalar loads into SGPRs, unpacked on the scalar unit,
+// operands of the FMAs): no LDS reads for them and no per-lane conversion.  Same operations in
+// the same order as a plain per-pixel loop: bias, then taps 0..8.  The block's 256 x 32 BF16
+// outputs (16 KB, contiguous in NHWC) go out through LDS so that every store instruction
+// writes 1 KB of consecutive bytes.  w2: the [32][9] BF16 weights as 144 words.
+__global__ __launch_bounds__(256) void k_conv_in(int H, int W, const float* __restrict__ x,
+                                                 const uint32_t* __restrict__ w2,
                                                  const float* __restrict__ bias,
                                                  __bf16* __restrict__ h) {
-  __shared__ float sw[NCH * 9];
-  __shared__ float sb[NCH];
-  for (int i = threadIdx.x; i < NCH * 9; i += 256) sw[i] = (float)w[i];
-  if (threadIdx.x < NCH) sb[threadIdx.x] = bias[threadIdx.x];
-  __syncthreads();
-  const long p = (long)blockIdx.x * 256 + threadIdx.x;
-  if (p >= npix) return;
-  const long b = p / ((long)H * W);
-  const int r = (int)(p - b * H * W);
+  __shared__ bf16x8 st[256 * 4];
+  const int r0 = blockIdx.x * 256;
+  const int r = r0 + threadIdx.x;
+  const bool live = r < H * W;
+  const size_t img = (size_t)blockIdx.y * H * W;
   const int py = r / W, px = r - py * W;
   float v[9];
+#pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-    v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[(b * H + yy) * W + xx] : 0.0f;
+    v[t] = (live && yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[img + (size_t)yy * W + xx] : 0.0f;
   }
-  __bf16* out = h + p * NCH;
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
     bf16x8 o;
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int co = q * 8 + i;
-      float s = sb[co];
-      for (int t = 0; t < 9; ++t) s = fmaf(sw[co * 9 + t], v[t], s);
+      float s = bias[co];
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int k = co * 9 + t;
+        const uint32_t pr = __builtin_amdgcn_readfirstlane(w2[k >> 1]);
+        s = fmaf(__builtin_bit_cast(float, (k & 1) ? (pr & 0xffff0000u) : (pr << 16)), v[t], s);
+      }
       o[i] = (__bf16)fmaxf(s, 0.0f);
     }
-    *(bf16x8*)(out + q * 8) = o;
+    st[threadIdx.x * 4 + q] = o;
+  }
+  __syncthreads();
+  bf16x8* out = (bf16x8*)(h + (img + r0) * NCH);
+  const int n = min(256, H * W - r0) * 4;  // 16-byte chunks of this block's pixels
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = threadIdx.x + 256 * k;
+    if (c < n) out[c] = st[c];
   }
 }
 
@@ -444,9 +462,10 @@ int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, 
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (B < 1 || H < 1 || W < 1 || !x || !w || !bias || !h) return fail(&ctx->c, SK_EINVAL, "conv_in: bad arguments");
-  const long npix = (long)B * H * W;
-  hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream, H, W,
-                     npix, x, (const __bf16*)w, bias, (__bf16*)h);
+  if (B > 65535 || (long)H * W >= (1L << 31)) return fail(&ctx->c, SK_EINVAL, "conv_in: batch or image too large");
+  if ((uintptr_t)w % 4) return fail(&ctx->c, SK_EINVAL, "conv_in: weights must be 4-byte aligned");
+  hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((H * W + 255) / 256), B), dim3(256), 0, (hipStream_t)stream, H, W,
+                     x, (const uint32_t*)w, bias, (__bf16*)h);
   return launched(&ctx->c, "conv_in kernel");
 }
 
