@@ -648,6 +648,42 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
     }
     __syncthreads();
   }
+  if constexpr (MODE == BP_SIRT) {
+    // SIRT.fut:37-39: x + C * B(q), written to x and to its transpose.  The transposed copy
+    // goes out through LDS (the window buffer, free after the line loop's last barrier) when
+    // the tile fits: each column of the tile becomes a contiguous run of x^T instead of a
+    // scatter with stride N.
+    constexpr int TS = 17;  // LDS row stride of the staged tile (floats)
+    float* st = reinterpret_cast<float*>(win);
+    const bool staged = 2 * BP_LB * W >= TH * TS;  // uniform
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int Y = Y0 + ty + 16 * q;
+      if (X >= N || Y >= N) continue;
+      const size_t p = (size_t)bz * N * N + (size_t)Y * N + X;
+      const float v = acc_s[q] + acc_f[q];  // backprojection.fut:78
+      const float xn = P.out[p] + P.colinv[(size_t)Y * N + X] * v;
+      P.out[p] = xn;
+      if (staged)
+        st[(ty + 16 * q) * TS + tx] = xn;
+      else
+        P.outT[(size_t)bz * N * N + (size_t)X * N + Y] = xn;
+    }
+    if (staged) {
+      __syncthreads();
+      // thread -> column x = tid / 16 of the tile, rows PPT*(tid % 16) .. + PPT-1
+      const int cx = threadIdx.x >> 4, cy = (threadIdx.x & 15) * PPT;
+      const int XT = X0 + cx;
+      if (XT < N) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+          const int YT = Y0 + cy + k;
+          if (YT < N) P.outT[(size_t)bz * N * N + (size_t)XT * N + YT] = st[(cy + k) * TS + cx];
+        }
+      }
+    }
+    return;
+  }
   if (X >= N) return;
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
@@ -657,13 +693,8 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
     const float v = acc_s[q] + acc_f[q];  // backprojection.fut:78
     if constexpr (MODE == BP_PLAIN) {
       P.out[p] = v;
-    } else if constexpr (MODE == BP_ONES_INV) {
-      P.out[p] = (v == 0.0f) ? 0.0f : 1.0f / v;
     } else {
-      // SIRT.fut:37-39: x + C * B(q)
-      const float xn = P.out[p] + P.colinv[(size_t)Y * N + X] * v;
-      P.out[p] = xn;
-      P.outT[(size_t)bz * N * N + (size_t)X * N + Y] = xn;
+      P.out[p] = (v == 0.0f) ? 0.0f : 1.0f / v;
     }
   }
 }
