@@ -390,6 +390,8 @@ __host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
 struct alignas(16) BpLineLds {
   int w0;
   float ci, si, cst;
+  float cost, lbase;  // the line's (window staging)
+  int orig, pad;
 };
 
 // One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
@@ -453,21 +455,34 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         e.si = L.sint * idr;
         e.cst = -(fmaf(L.cost, L.fc, L.sint * L.fs) + P.rho0) * idr;
       }
+      e.cost = L.cost;
+      e.lbase = L.lbase;
+      e.orig = L.orig;
+      e.pad = 0;
       lsh[threadIdx.x] = e;
     }
     __syncthreads();
-    // One wave per line (line values in scalar registers), lanes over the window's bins.
-    for (int jj = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); jj < nl; jj += 4) {
-      const Line L = P.lines[l0 + jj];
-      const int w0 = lsh[jj].w0;
-      const float* srow = sb + (size_t)L.orig * P.R;
-      for (int k = threadIdx.x & 63; k < W; k += 64) {
-        const int sp = w0 + k;
+    // Window staging, flattened over (line, bin): the 256 threads take entries e = jj*W + k of
+    // the block's nl windows in turn (a wave per line would leave most lanes of its second
+    // pass over a 70-bin window idle).  e / W by a multiply-high: exact for e < 2^32 / W.
+    {
+      const int total = nl * W;
+      const unsigned Mw = 0xffffffffu / (unsigned)W + 1u;
+      for (int e = threadIdx.x; e < total; e += 256) {
+        const int jj = (int)__umulhi((unsigned)e, Mw);
+        const int k = e - jj * W;
+        const int sp = lsh[jj].w0 + k;
         // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost
-        const float inter = ((float)sp * P.drho + P.rho0) / L.cost;
-        float p = 0.0f;
-        if (sp >= 0 && sp < P.R) p = (MODE == BP_ONES_INV) ? 1.0f : srow[sp];
-        if constexpr (FASTREF) p = p * L.lbase;
+        const float inter = ((float)sp * P.drho + P.rho0) / lsh[jj].cost;
+        const bool in = sp >= 0 && sp < P.R;
+        float p;
+        if constexpr (MODE == BP_ONES_INV) {
+          p = in ? 1.0f : 0.0f;
+        } else {
+          const float v = sb[(size_t)lsh[jj].orig * P.R + (in ? sp : 0)];  // in bounds either way
+          p = in ? v : 0.0f;
+        }
+        if constexpr (FASTREF) p = p * lsh[jj].lbase;
         win[jj * W + (REV ? W - 1 - k : k)] = make_float2(inter, p);
       }
     }
