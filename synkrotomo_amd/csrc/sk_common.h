@@ -43,6 +43,9 @@ struct Geom {
   int fps_w = 0;
   std::vector<Line> lines;   // host copy, steep lines then flat lines
   Line* d_lines = nullptr;
+  // backprojector intercepts (s*drho + rho0)/cost per table line j and bin s, [A][R] (null when
+  // A*R >= 2^31: the kernels then divide)
+  float* d_inter = nullptr;
   // SIRT weights, computed on first use (SIRT.fut:22-25)
   float* d_rowinv = nullptr;  // [A][R]
   float* d_colinv = nullptr;  // [N][N]

@@ -87,6 +87,15 @@ static int build_lines(const float* ang, int A, std::vector<Line>& out) {
   return (int)steep.size();
 }
 
+// inter[j][s] = (s*drho + rho0) / cost_j (backprojection.fut:57-58), IEEE division: the values
+// the backprojector stages for its candidate windows.
+__global__ __launch_bounds__(256) void k_inter_table(const Line* __restrict__ lines, int R, float rho0,
+                                                     float drho, float* __restrict__ out) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= R) return;
+  out[(size_t)blockIdx.y * R + s] = ((float)s * drho + rho0) / lines[blockIdx.y].cost;
+}
+
 }  // namespace sk
 
 using namespace sk;
@@ -198,8 +207,18 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   hipError_t e = hipMalloc(&G.d_lines, sizeof(Line) * (size_t)A);
   if (e == hipSuccess)
     e = hipMemcpy(G.d_lines, G.lines.data(), sizeof(Line) * (size_t)A, hipMemcpyHostToDevice);
+  if (e == hipSuccess && (long)A * R < (1L << 31) && A <= 65535) {
+    e = hipMalloc(&G.d_inter, sizeof(float) * (size_t)A * R);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_inter_table, dim3((R + 255) / 256, A), dim3(256), 0, nullptr, G.d_lines, R, rho0, drho,
+                         G.d_inter);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  }
   if (e != hipSuccess) {
     if (G.d_lines) (void)hipFree(G.d_lines);
+    if (G.d_inter) (void)hipFree(G.d_inter);
     delete g;
     return hip_fail(c, e, "geometry upload");
   }
@@ -214,6 +233,7 @@ void sk_geom_destroy(sk_geom* g) {
   for (auto& sg : g->g.sirt_graphs)
     if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   if (g->g.d_lines) (void)hipFree(g->g.d_lines);
+  if (g->g.d_inter) (void)hipFree(g->g.d_inter);
   if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
   if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);
   delete g;

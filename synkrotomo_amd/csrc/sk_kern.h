@@ -372,6 +372,7 @@ struct BpParams {
   float* out;           // [batch][N][N]  (BP_SIRT: the image, updated in place)
   float* outT;          // BP_SIRT: transposed copy of the updated image
   const float* colinv;  // BP_SIRT: [N][N]
+  const float* inter;   // [A][R] (s*drho + rho0)/cost per table line, or null
 };
 
 // LDS window width (bins) covering every candidate of a BP_TW x (16*PPT) tile.
@@ -472,9 +473,14 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         const int jj = (int)__umulhi((unsigned)e, Mw);
         const int k = e - jj * W;
         const int sp = lsh[jj].w0 + k;
-        // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost
-        const float inter = ((float)sp * P.drho + P.rho0) / lsh[jj].cost;
         const bool in = sp >= 0 && sp < P.R;
+        // backprojection.fut:57-58: r = s'*drho + rho0, then r/cost (from the geometry's table
+        // for bins of the sinogram; bins outside it are rare: divide)
+        float inter;
+        if (in && P.inter)
+          inter = P.inter[(l0 + jj) * P.R + sp];
+        else
+          inter = ((float)sp * P.drho + P.rho0) / lsh[jj].cost;
         float p;
         if constexpr (MODE == BP_ONES_INV) {
           p = in ? 1.0f : 0.0f;

@@ -182,7 +182,7 @@ int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s) {
 
 BpParams bp_params(const Geom* g, const BpArgs& a, int ppt) {
   return BpParams{g->d_lines, g->A, g->R, g->N, g->ncand, bp_window_bins(ppt, g->drho, g->ncand),
-                  g->n_steep, g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv};
+                  g->n_steep, g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv, g->d_inter};
 }
 
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
