@@ -217,10 +217,11 @@ int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols
  * the gathers of steep rays (whose pixels step through image rows) from a transposed copy of
  * the input, where they are contiguous; 7 = one lane per row on a SELL-64 copy of the matrix
  * (64-row slices stored column-major, both streams coalesced; built on first use, about the
- * size of the CSR arrays).  Kernels that sum a row in the same order give the same bits:
- * 0, 3; 2, 4; 1, 7.  -1 (default) = time the candidates (0, 2; 3, 4 for SK_MAT_FP; 7 if its
- * copy fits in half the free device memory) on the first sk_spmv call and keep the fastest.
- * EINVAL for 5, 6, and for 3, 4 on other matrices. */
+ * size of the CSR arrays); SK_MAT_FP only: 8 = as 7 with the flat rays' gathers from the
+ * transposed input.  Kernels that sum a row in the same order give the same bits: 0, 3;
+ * 2, 4; 1, 7, 8.  -1 (default) = time the candidates (0, 2; 3, 4 for SK_MAT_FP; 7, and 8 for
+ * SK_MAT_FP, if the copy fits in half the free device memory) on the first sk_spmv call and
+ * keep the fastest.  EINVAL for 5, 6, and for 3, 4, 8 on other matrices. */
 int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 /* d_out[batch][rows] = M . d_in[batch][cols] */
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);

@@ -436,40 +436,76 @@ __global__ __launch_bounds__(256) void k_sell_fill(const int64_t* __restrict__ r
 }
 
 // One wave per slice, one lane per row, entries in CSR order (the lockstep kernel's order and
-// operations: bitwise equal to variant 1); entries past a row's end are skipped.
-template <int U>
+// operations: bitwise equal to variant 1); entries past a row's end are skipped.  TRF (A only):
+// the rows of flat rays gather from x^T -- the 64 parallel rays of a slice then read
+// neighbouring pixels at each step for steep and flat angles alike.
+template <int U, bool TRF>
 __global__ __launch_bounds__(256) void k_spmv_sell(const int64_t* __restrict__ off,
                                                    const int32_t* __restrict__ scol,
                                                    const float* __restrict__ sval,
                                                    const int32_t* __restrict__ len, long rows,
                                                    long slices, long ncols, const float* __restrict__ x,
-                                                   float* __restrict__ y) {
+                                                   float* __restrict__ y, const float* __restrict__ xT,
+                                                   const unsigned char* __restrict__ steep, int R, int N,
+                                                   double invN) {
   const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (slice >= slices) return;
   const int lane = threadIdx.x & 63;
   const long row = slice * 64 + lane;
   const int n = row < rows ? len[row] : 0;
-  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const bool tr = TRF && row < rows && !steep[row / R];
+  const float* xb = (tr ? xT : x) + (size_t)blockIdx.y * ncols;
+  auto at = [&](int c) -> float {
+    if (!tr) return xb[c];
+    const int yq = (int)(((double)c + 0.5) * invN);  // c = yq*N + xq, exact (see SpGather)
+    return xb[(c - yq * N) * N + yq];
+  };
   const long base = off[slice];
   const int L = (int)((off[slice + 1] - base) >> 6);
   const int32_t* c = scol + base + lane;
   const float* v = sval + base + lane;
   float acc = 0.0f;
   int j = 0;
-  for (; j + U <= L; j += U) {
+  if (U <= L) {
+    // Software-pipelined: the next block's column and value loads are issued after this
+    // block's gathers, so they are in flight while the gathered values are summed.
     int cc[U];
     float vv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      cc[u] = __builtin_nontemporal_load(c + 64 * (j + u));
-      vv[u] = __builtin_nontemporal_load(v + 64 * (j + u));
+      cc[u] = __builtin_nontemporal_load(c + 64 * u);
+      vv[u] = __builtin_nontemporal_load(v + 64 * u);
     }
+    for (; j + U <= L; j += U) {
+      float g[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (j + u < n) acc = fmaf(vv[u], xb[cc[u]], acc);
+      for (int u = 0; u < U; ++u) g[u] = at(cc[u]);  // padding columns are 0: in bounds
+      const bool more = j + 2 * U <= L;  // uniform over the wave
+      int cn[U];
+      float vn[U];
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          cn[u] = __builtin_nontemporal_load(c + 64 * (j + U + u));
+          vn[u] = __builtin_nontemporal_load(v + 64 * (j + U + u));
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < n) acc = fmaf(vv[u], g[u], acc);
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          cc[u] = cn[u];
+          vv[u] = vn[u];
+        }
+      }
+    }
   }
   for (; j < L; ++j)
-    if (j < n) acc = fmaf(v[64 * j], xb[c[64 * j]], acc);
+    if (j < n) acc = fmaf(v[64 * j], at(c[64 * j]), acc);
   if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
 }
 
@@ -718,8 +754,8 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 7 || variant == 5 || variant == 6) return SK_EINVAL;
-  if ((variant == 3 || variant == 4) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 8 || variant == 5 || variant == 6) return SK_EINVAL;
+  if ((variant == 3 || variant == 4 || variant == 8) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -782,12 +818,21 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   const double invN = m->N > 0 ? 1.0 / (double)m->N : 0.0;
   auto run = [&](int v) -> int {
     dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-    if (v == 7) {
+    if (v == 7 || v == 8) {
       int rc = build_sell(c, const_cast<sk_mat*>(m), true, s);
       if (rc) return rc;
       const long slices = (m->rows + 63) / 64;
-      hipLaunchKernelGGL(k_spmv_sell<4>, dim3((unsigned)((slices + 3) / 4), batch), dim3(256), 0, s, m->sell_off,
-                         m->sell_col, m->sell_val, m->sell_len, (long)m->rows, slices, (long)m->cols, d_in, d_out);
+      const dim3 g7((unsigned)((slices + 3) / 4), batch);
+      if (v == 7) {
+        hipLaunchKernelGGL((k_spmv_sell<4, false>), g7, dim3(256), 0, s, m->sell_off, m->sell_col, m->sell_val,
+                           m->sell_len, (long)m->rows, slices, (long)m->cols, d_in, d_out, nullptr, nullptr, 1, 0, 0.0);
+      } else {
+        rc = need_xT();
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_spmv_sell<4, true>), g7, dim3(256), 0, s, m->sell_off, m->sell_col, m->sell_val,
+                           m->sell_len, (long)m->rows, slices, (long)m->cols, d_in, d_out, xT, m->steep, m->R, m->N,
+                           invN);
+      }
     } else if (v == 3 || v == 4) {
       int rc = need_xT();
       if (rc) return rc;
@@ -823,10 +868,11 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     const bool sell = m->sell_col != nullptr;
     // (8 loads in flight per lane instead of 4 measured no faster: 45.8 vs 43.9 ms for A,
     // 63.5 vs 62.6 ms for B at 2048^2.)
-    int cand[5] = {0, 2, 3, 4, 7};
+    int cand[6] = {0, 2, 3, 4, 7, 8};
     int NV = fpT ? 4 : 2;
     if (sell) cand[NV++] = 7;
-    hipEvent_t e[6];
+    if (sell && fpT) cand[NV++] = 8;
+    hipEvent_t e[7];
     for (int v = 0; v <= NV; ++v) SK_HIP(c, hipEventCreate(&e[v]));
     if (fpT) {
       int rc = need_xT();  // outside the timed region of variants 3, 4 (as for every later call)

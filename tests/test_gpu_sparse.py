@@ -64,21 +64,22 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
 def test_spmv_variants_bitwise(ctx, name, N, ang, R, rz, dr):
     """Every SpMV kernel that sums a row in the same order gives the same bits, batched too:
     wave per row 0 == 3 (A: steep rays gather from x^T); 16-byte streams 2 == 4 (x^T); lane
-    per row 1 == 7 (SELL-64 copy).  B rejects 3 and 4; 5 and 6 do not exist."""
+    per row 1 == 7 (SELL-64 copy) == 8 (A: SELL-64, flat rays gather from x^T).  B rejects 3,
+    4 and 8; 5 and 6 do not exist."""
     rng = np.random.default_rng(13)
     geo = ctx.geometry(ang, rz, dr, R, N)
     for kind, ncols in (("fp", N * N), ("bp", ang.size * R)):
         x = _dev(rng.random((3, ncols)).astype(np.float32))
         M = geo.sparse(kind)
-        variants = (0, 1, 2, 3, 4, 7) if kind == "fp" else (0, 1, 2, 7)
+        variants = (0, 1, 2, 3, 4, 7, 8) if kind == "fp" else (0, 1, 2, 7)
         out = {}
         for v in variants:
             M.set_spmv_variant(v)
             out[v] = M.spmv(x).cpu().numpy()
-        same = [(1, 7)] + ([(0, 3), (2, 4)] if kind == "fp" else [])
+        same = [(1, 7)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else [])
         for a, b in same:
             np.testing.assert_array_equal(out[b], out[a], err_msg=f"{kind}: variant {b} vs {a}")
-        for v in ((3, 4, 5, 6) if kind == "bp" else (5, 6, 8)):
+        for v in ((3, 4, 5, 6, 8) if kind == "bp" else (5, 6, 9)):
             with pytest.raises(ValueError):
                 M.set_spmv_variant(v)
     geo.drop_sparse()
