@@ -223,6 +223,8 @@ int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols
  * SK_MAT_FP, if the copy fits in half the free device memory) on the first sk_spmv call and
  * keep the fastest.  EINVAL for 5, 6, and for 3, 4, 8 on other matrices. */
 int sk_mat_set_spmv_variant(sk_mat* m, int variant);
+/* The SpMV kernel in use (-1 before the first sk_spmv call of an untimed matrix). */
+int sk_mat_spmv_variant(const sk_mat* m);
 /* d_out[batch][rows] = M . d_in[batch][cols] */
 int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d_out, void* stream);
 void sk_mat_destroy(sk_mat* m);

@@ -61,6 +61,7 @@ SIGNATURES = {
     "sk_mat_arrays": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "sk_mat_copy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sk_mat_set_spmv_variant": (_i, [_vp, _i]),
+    "sk_mat_spmv_variant": (_i, [_vp]),
     "sk_spmv": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_mat_destroy": (None, [_vp]),
     "sk_normalize": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp, _f, _f, _vp, _vp]),

@@ -754,6 +754,8 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
   return rc;
 }
 
+int sk_mat_spmv_variant(const sk_mat* m) { return m ? m->spmv_variant : -1; }
+
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
   if (!m || variant < -1 || variant > 8 || variant == 5 || variant == 6) return SK_EINVAL;
   if ((variant == 3 || variant == 4 || variant == 8) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;

@@ -311,6 +311,11 @@ class Matrix:
                                         _stream(self.geom.ctx.device)), self.geom.ctx.handle, "matrix copy")
         return rp, c[: self.nnz], v[: self.nnz]
 
+    @property
+    def spmv_variant(self) -> int:
+        """The SpMV kernel in use (-1 until the first spmv call times the candidates)."""
+        return int(nat.lib().sk_mat_spmv_variant(self.handle))
+
     def set_spmv_variant(self, variant: int) -> None:
         """0: wave per row, 1: lane per row (lockstep), 2: wave per row with 16-byte streams,
         3 / 4: as 0 / 2 with the steep rays' gathers from x^T (A only), 7: lane per row on a

@@ -75,6 +75,7 @@ def test_spmv_variants_bitwise(ctx, name, N, ang, R, rz, dr):
         out = {}
         for v in variants:
             M.set_spmv_variant(v)
+            assert M.spmv_variant == v
             out[v] = M.spmv(x).cpu().numpy()
         same = [(1, 7)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else [])
         for a, b in same:
