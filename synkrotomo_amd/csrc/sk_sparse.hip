@@ -415,23 +415,41 @@ __global__ __launch_bounds__(256) void k_sell_sizes(const int64_t* __restrict__ 
   if ((threadIdx.x & 63) == 0 && row < rows) slice_entries[row >> 6] = 64 * m;
 }
 
-// One wave per slice: lane l copies row 64k + l's entries into columns j = 0 .. L_k - 1.
-__global__ __launch_bounds__(256) void k_sell_fill(const int64_t* __restrict__ rowptr,
-                                                   const int32_t* __restrict__ col,
-                                                   const float* __restrict__ val, long rows,
-                                                   long slices, const int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ scol, float* __restrict__ sval) {
-  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (slice >= slices) return;
-  const int lane = threadIdx.x & 63;
-  const long row = slice * 64 + lane;
-  const long beg = row < rows ? rowptr[row] : 0;
-  const long n = row < rows ? rowptr[row + 1] - beg : 0;
-  const long L = (off[slice + 1] - off[slice]) >> 6;
-  for (long j = 0; j < L; ++j) {
-    const bool in = j < n;
-    scol[off[slice] + 64 * j + lane] = in ? col[beg + j] : 0;
-    sval[off[slice] + 64 * j + lane] = in ? val[beg + j] : 0.0f;
+// One one-wave block per slice, through a 64 x 64 LDS tile: for entries j0 .. j0+63, the wave
+// reads each of the slice's rows coalesced (lane = entry), then writes the tile transposed
+// (lane = row), so both the CSR reads and the SELL writes are contiguous.
+__global__ __launch_bounds__(64) void k_sell_fill(const int64_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col,
+                                                  const float* __restrict__ val, long rows,
+                                                  const int64_t* __restrict__ off,
+                                                  int32_t* __restrict__ scol, float* __restrict__ sval) {
+  __shared__ int32_t tc[64][65];
+  __shared__ float tv[64][65];
+  __shared__ long sbeg[64];
+  __shared__ int slen[64];
+  const long slice = blockIdx.x;
+  const int lane = threadIdx.x;
+  const long myrow = slice * 64 + lane;
+  sbeg[lane] = myrow < rows ? rowptr[myrow] : 0;
+  slen[lane] = myrow < rows ? (int)(rowptr[myrow + 1] - rowptr[myrow]) : 0;
+  const long base = off[slice];
+  const long L = (off[slice + 1] - base) >> 6;
+  __syncthreads();
+  for (long j0 = 0; j0 < L; j0 += 64) {
+    const long j = j0 + lane;
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r) {
+      const bool in = j < slen[r];
+      tc[r][lane] = in ? col[sbeg[r] + j] : 0;
+      tv[r][lane] = in ? val[sbeg[r] + j] : 0.0f;
+    }
+    __syncthreads();
+    const int nj = (int)min(64L, L - j0);
+    for (int jj = 0; jj < nj; ++jj) {
+      scol[base + (j0 + jj) * 64 + lane] = tc[lane][jj];
+      sval[base + (j0 + jj) * 64 + lane] = tv[lane][jj];
+    }
+    __syncthreads();
   }
 }
 
@@ -652,8 +670,11 @@ static int build_sell(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
     m->sell_failed = true;
     return required ? hip_fail(c, e, "SELL copy allocation") : SK_OK;
   }
-  hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)((slices + 3) / 4)), dim3(256), 0, s, m->rowptr, m->col, m->val,
-                     (long)m->rows, slices, m->sell_off, m->sell_col, m->sell_val);
+  if (getenv("SYNKRO_VERBOSE"))
+    fprintf(stderr, "sk_spmv: SELL-64 copy, %lld entries for %lld nonzeros (%.1f%% padding)\n", (long long)total,
+            (long long)m->nnz, 100.0 * (double)(total - m->nnz) / (double)std::max<int64_t>(m->nnz, 1));
+  hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)slices), dim3(64), 0, s, m->rowptr, m->col, m->val,
+                     (long)m->rows, m->sell_off, m->sell_col, m->sell_val);
   return launched(c, "SELL fill kernel");
 }
 
