@@ -218,10 +218,15 @@ int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols
  * the input, where they are contiguous; 7 = one lane per row on a SELL-64 copy of the matrix
  * (64-row slices stored column-major, both streams coalesced; built on first use, about the
  * size of the CSR arrays); SK_MAT_FP only: 8 = as 7 with the flat rays' gathers from the
- * transposed input.  Kernels that sum a row in the same order give the same bits: 0, 3;
- * 2, 4; 1, 7, 8.  -1 (default) = time the candidates (0, 2; 3, 4 for SK_MAT_FP; 7, and 8 for
- * SK_MAT_FP, if the copy fits in half the free device memory) on the first sk_spmv call and
- * keep the fastest.  EINVAL for 5, 6, and for 3, 4, 8 on other matrices. */
+ * transposed input; 9 = as 7 with the columns stored as 16-bit steps from the previous entry
+ * of the row (6 bytes per entry instead of 8; for SK_MAT_BP / SK_MAT_ADJ the steps run over the
+ * rays in line-table order, steep lines then flat lines, and each call first permutes the
+ * input's rows into that order; unavailable, and sk_spmv fails with EINVAL, when two
+ * consecutive columns of a row are more than 32767 apart).  Kernels that sum a row in the
+ * same order give the same bits: 0, 3; 2, 4; 1, 7, 8, 9.  -1 (default) = time the candidates
+ * (0, 2; 3, 4 for SK_MAT_FP; 7, 8 for SK_MAT_FP and 9, if their copies fit in half the free
+ * device memory) on the first sk_spmv call and keep the fastest.  EINVAL for 5, 6, and for 3,
+ * 4, 8 on other matrices. */
 int sk_mat_set_spmv_variant(sk_mat* m, int variant);
 /* The SpMV kernel in use (-1 before the first sk_spmv call of an untimed matrix). */
 int sk_mat_spmv_variant(const sk_mat* m);

@@ -36,7 +36,8 @@ struct sk_mat {
   int kind = 0;
   int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
                           // 16-byte streams; 3, 4: as 0, 2 with the gathers of steep rays
-                          // from x^T (SK_MAT_FP only); 7: SELL-64 copy; -1: not yet timed
+                          // from x^T (SK_MAT_FP only); 7: SELL-64 copy; 9: SELL-64 with
+                          // 16-bit column steps; -1: not yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   // SK_MAT_FP: per original angle, 1 if its line is steep (its rays step through image rows:
   // in x^T their pixels are contiguous), and the geometry's R and N
@@ -51,6 +52,21 @@ struct sk_mat {
   float* sell_val = nullptr;
   int32_t* sell_len = nullptr;  // [rows] row lengths
   bool sell_failed = false;     // no memory for the copy: variant 7 unavailable
+  // Variant 9: the SELL-64 values with the columns as 16-bit steps d_j = col_j - col_{j-1}
+  // (col_{-1} = the row's first column, kept in sell_first; padding repeats the last column,
+  // d = 0), two entries per 32-bit word: word sell_off[k]/2 + 64*(j/2) + l, entry j in bits
+  // 16*(j&1) ... 16*(j&1)+15.  6 bytes per entry instead of 8.  Unavailable (d16_failed) when a
+  // row has a step outside [-32768, 32767] or there is no memory for it.
+  // For SK_MAT_BP / SK_MAT_ADJ the steps run over columns in line-table order, j*R + r for the
+  // line at table position j (a pixel's entries come steep lines first, then flat lines, so in
+  // the caller's angle order its columns jump by up to A*R once or twice; in table order each
+  // step is about R); sk_spmv permutes the input's rows into that order first.
+  uint32_t* sell_d16 = nullptr;
+  int32_t* sell_first = nullptr;  // [rows]
+  bool d16_failed = false;
+  int32_t* line_pos = nullptr;    // SK_MAT_BP / SK_MAT_ADJ: [A] table position of each angle,
+  int32_t* line_orig = nullptr;   //   then [A] the angle at each table position
+  int A = 0;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
   float* val = nullptr;       // [nnz]
@@ -403,7 +419,7 @@ __device__ __forceinline__ float spmv_row_wide(const int32_t* __restrict__ col, 
 }
 
 // ------------------------------------------------------------------ SELL-64 (variant 7)
-// Per slice: 64 * (longest row) entries; per row: its length.
+// Per slice: 64 * (longest row, rounded up to even) entries; per row: its length.
 __global__ __launch_bounds__(256) void k_sell_sizes(const int64_t* __restrict__ rowptr, long rows,
                                                     int32_t* __restrict__ slice_entries,
                                                     int32_t* __restrict__ len) {
@@ -412,6 +428,7 @@ __global__ __launch_bounds__(256) void k_sell_sizes(const int64_t* __restrict__ 
   if (row < rows) len[row] = n;
   int m = n;
   for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  m = (m + 1) & ~1;  // even: the 16-bit steps of variant 9 pack two entries per word
   if ((threadIdx.x & 63) == 0 && row < rows) slice_entries[row >> 6] = 64 * m;
 }
 
@@ -528,6 +545,136 @@ __global__ __launch_bounds__(256) void k_spmv_sell(const int64_t* __restrict__ o
   if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
 }
 
+// ------------------------------------------------------------------ 16-bit steps (variant 9)
+__device__ inline int step_lo(uint32_t w) { return (int)(int16_t)(w & 0xffffu); }
+__device__ inline int step_hi(uint32_t w) { return (int)w >> 16; }
+
+// One wave per slice, one lane per row: the SELL-64 columns of a row as 16-bit steps (layout
+// at sk_mat::sell_d16).  Sets *bad when a step does not fit.
+// pos (B, A^T): columns a*R + r are taken as pos[a]*R + r, their line-table order.
+__global__ __launch_bounds__(256) void k_sell_steps(const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ scol,
+                                                    const int32_t* __restrict__ len, long rows, long slices,
+                                                    const int32_t* __restrict__ pos, int R,
+                                                    uint32_t* __restrict__ d16, int32_t* __restrict__ first,
+                                                    unsigned* __restrict__ bad) {
+  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slice >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const long row = slice * 64 + lane;
+  const int n = row < rows ? len[row] : 0;
+  const long base = off[slice];
+  const int L = (int)((off[slice + 1] - base) >> 6);  // even
+  const int32_t* c = scol + base + lane;
+  uint32_t* d = d16 + (base >> 1) + lane;
+  auto colm = [&](int j) -> long {
+    const long v = c[64 * (long)j];
+    if (!pos) return v;
+    const long a = v / R;
+    return (long)pos[a] * R + (v - a * R);
+  };
+  long prev = n > 0 ? colm(0) : 0;
+  if (row < rows) first[row] = (int32_t)prev;
+  bool over = false;
+  for (int j = 0; j < L; j += 2) {
+    const long c0 = j < n ? colm(j) : prev;
+    const long c1 = j + 1 < n ? colm(j + 1) : c0;
+    const long d0 = c0 - prev, d1 = c1 - c0;
+    over = over || d0 < -32768 || d0 > 32767 || d1 < -32768 || d1 > 32767;
+    d[64 * (long)(j >> 1)] = (uint32_t)(uint16_t)(int16_t)d0 | ((uint32_t)(uint16_t)(int16_t)d1 << 16);
+    prev = c1;
+  }
+  if (over) atomicOr(bad, 1u);
+}
+
+// As k_spmv_sell<U, false> (the same entries, gathers and FMAs in the same order: bitwise
+// equal to variants 7 and 1), with the columns rebuilt from the 16-bit steps by a running sum
+// per lane.  Padding steps are 0, so its gathers stay on the row's last column: in bounds.
+template <int U>
+__global__ __launch_bounds__(256) void k_spmv_sell16(const int64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ d16,
+                                                     const float* __restrict__ sval,
+                                                     const int32_t* __restrict__ len,
+                                                     const int32_t* __restrict__ first, long rows, long slices,
+                                                     long ncols, const float* __restrict__ x,
+                                                     float* __restrict__ y) {
+  static_assert(U % 2 == 0, "two entries per step word");
+  constexpr int H = U / 2;
+  const long slice = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slice >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const long row = slice * 64 + lane;
+  const int n = row < rows ? len[row] : 0;
+  const float* xb = x + (size_t)blockIdx.y * ncols;
+  const long base = off[slice];
+  const int L = (int)((off[slice + 1] - base) >> 6);  // even
+  const uint32_t* dp = d16 + (base >> 1) + lane;
+  const float* v = sval + base + lane;
+  int cur = row < rows ? first[row] : 0;
+  float acc = 0.0f;
+  int j = 0;
+  if (U <= L) {
+    uint32_t ww[H];
+    float vv[U];
+#pragma unroll
+    for (int h = 0; h < H; ++h) ww[h] = __builtin_nontemporal_load(dp + 64 * h);
+#pragma unroll
+    for (int u = 0; u < U; ++u) vv[u] = __builtin_nontemporal_load(v + 64 * u);
+    for (; j + U <= L; j += U) {
+      float g[U];
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        cur += step_lo(ww[h]);
+        g[2 * h] = xb[cur];
+        cur += step_hi(ww[h]);
+        g[2 * h + 1] = xb[cur];
+      }
+      const bool more = j + 2 * U <= L;  // uniform over the wave
+      uint32_t wn[H];
+      float vn[U];
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) wn[h] = __builtin_nontemporal_load(dp + 64 * ((j + U) / 2 + h));
+#pragma unroll
+        for (int u = 0; u < U; ++u) vn[u] = __builtin_nontemporal_load(v + 64 * (j + U + u));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u < n) acc = fmaf(vv[u], g[u], acc);
+      if (more) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) ww[h] = wn[h];
+#pragma unroll
+        for (int u = 0; u < U; ++u) vv[u] = vn[u];
+      }
+    }
+  }
+  for (; j < L; j += 2) {
+    const uint32_t w = dp[64 * (j >> 1)];
+    cur += step_lo(w);
+    const float g0 = xb[cur];
+    cur += step_hi(w);
+    const float g1 = xb[cur];
+    if (j < n) acc = fmaf(v[64 * j], g0, acc);
+    if (j + 1 < n) acc = fmaf(v[64 * (j + 1)], g1, acc);
+  }
+  if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
+}
+
+// out row b*A + j = in row b*A + orig[j] (R floats each): sinograms into line-table order.
+// One block per row.
+__global__ __launch_bounds__(256) void k_rows_permute(const float* __restrict__ in, const int32_t* __restrict__ orig,
+                                                      int A, int R, float* __restrict__ out) {
+  const long row = blockIdx.x;
+  const long b = row / A;
+  const int j = (int)(row - b * A);
+  const float* src = in + (b * A + orig[j]) * (long)R;
+  float* dst = out + row * (long)R;
+  for (int r = threadIdx.x; r < R; r += 256) dst[r] = src[r];
+}
+
 // counts -> rowptr (rows + 1 entries).  Returns nnz via host sync.
 static int scan_counts(Ctx* c, const int32_t* counts, long rows, int64_t* rowptr, int64_t* nnz,
                        hipStream_t s) {
@@ -628,6 +775,9 @@ static void mat_free(sk_mat* m) {
   if (m->sell_col) (void)hipFree(m->sell_col);
   if (m->sell_val) (void)hipFree(m->sell_val);
   if (m->sell_len) (void)hipFree(m->sell_len);
+  if (m->sell_d16) (void)hipFree(m->sell_d16);
+  if (m->sell_first) (void)hipFree(m->sell_first);
+  if (m->line_pos) (void)hipFree(m->line_pos);
   delete m;
 }
 
@@ -678,6 +828,60 @@ static int build_sell(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
   return launched(c, "SELL fill kernel");
 }
 
+// The 16-bit steps of variant 9, from the SELL-64 copy (built first if needed).  With
+// `required` false it is built only if it takes at most half the free device memory.  A matrix
+// with a step beyond 16 bits (consecutive columns of a row more than 32767 apart) marks it
+// unavailable.  Synchronizes `s`.
+static int build_d16(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
+  if (m->sell_d16) return SK_OK;
+  auto unavailable = [&]() {
+    m->d16_failed = true;
+    return required ? fail(c, SK_EINVAL,
+                           "SpMV variant 9 unavailable: a row steps more than 32767 columns (or no memory)")
+                    : SK_OK;
+  };
+  if (m->d16_failed) return unavailable();
+  int rc = build_sell(c, m, required, s);
+  if (rc) return rc;
+  if (!m->sell_col) return unavailable();
+  const long slices = (m->rows + 63) / 64;
+  int64_t total = 0;
+  SK_HIP(c, hipMemcpyAsync(&total, m->sell_off + slices, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SK_HIP(c, hipStreamSynchronize(s));
+  size_t free_b = 0, total_b = 0;
+  SK_HIP(c, hipMemGetInfo(&free_b, &total_b));
+  const size_t need = sizeof(uint16_t) * (size_t)total + sizeof(int32_t) * (size_t)m->rows;
+  if (!required && need > free_b / 2) return unavailable();
+  unsigned* bad = nullptr;
+  hipError_t e = hipMalloc(&m->sell_d16, sizeof(uint16_t) * (size_t)std::max<int64_t>(total, 2));
+  if (e == hipSuccess) e = hipMalloc(&m->sell_first, sizeof(int32_t) * (size_t)m->rows);
+  if (e == hipSuccess) e = hipMalloc(&bad, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(unsigned), s);
+  unsigned hbad = 1;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sell_steps, dim3((unsigned)((slices + 3) / 4)), dim3(256), 0, s, m->sell_off,
+                       m->sell_col, m->sell_len, (long)m->rows, slices, m->line_pos, m->R, m->sell_d16,
+                       m->sell_first, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (bad) (void)hipFree(bad);
+  if (e != hipSuccess || hbad) {
+    if (e != hipSuccess) (void)hipGetLastError();
+    if (m->sell_d16) (void)hipFree(m->sell_d16);
+    if (m->sell_first) (void)hipFree(m->sell_first);
+    m->sell_d16 = nullptr;
+    m->sell_first = nullptr;
+    if (e != hipSuccess && required) {
+      m->d16_failed = true;
+      return hip_fail(c, e, "SpMV step copy");
+    }
+    return unavailable();
+  }
+  return SK_OK;
+}
+
 extern "C" {
 
 int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stream) {
@@ -709,7 +913,21 @@ int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stre
       mat_free(m);
       return hip_fail(c, e, "steep-angle table");
     }
+  } else {
+    std::vector<int32_t> t(2 * (size_t)G.A, 0);
+    for (size_t j = 0; j < G.lines.size(); ++j) {
+      t[(size_t)G.lines[j].orig] = (int32_t)j;
+      t[(size_t)G.A + j] = G.lines[j].orig;
+    }
+    e = hipMalloc(&m->line_pos, sizeof(int32_t) * t.size());
+    if (e == hipSuccess) e = hipMemcpy(m->line_pos, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      mat_free(m);
+      return hip_fail(c, e, "line-order table");
+    }
+    m->line_orig = m->line_pos + G.A;
   }
+  m->A = G.A;
   e = hipMalloc(&m->rowptr, sizeof(int64_t) * (size_t)(m->rows + 1));
   if (e != hipSuccess) {
     mat_free(m);
@@ -778,7 +996,7 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 int sk_mat_spmv_variant(const sk_mat* m) { return m ? m->spmv_variant : -1; }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 8 || variant == 5 || variant == 6) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 9 || variant == 5 || variant == 6) return SK_EINVAL;
   if ((variant == 3 || variant == 4 || variant == 8) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
@@ -842,7 +1060,23 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   const double invN = m->N > 0 ? 1.0 / (double)m->N : 0.0;
   auto run = [&](int v) -> int {
     dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-    if (v == 7 || v == 8) {
+    if (v == 9) {
+      int rc = build_d16(c, const_cast<sk_mat*>(m), true, s);
+      if (rc) return rc;
+      const long slices = (m->rows + 63) / 64;
+      const float* in = d_in;
+      if (m->line_orig) {  // B, A^T: the input's rows in line-table order
+        void* ws = nullptr;
+        rc = scratch(c, sizeof(float) * (size_t)batch * m->cols, s, &ws);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_rows_permute, dim3((unsigned)((long)batch * m->A)), dim3(256), 0, s, d_in,
+                           m->line_orig, m->A, m->R, (float*)ws);
+        in = (const float*)ws;
+      }
+      hipLaunchKernelGGL(k_spmv_sell16<8>, dim3((unsigned)((slices + 3) / 4), batch), dim3(256), 0, s,
+                         m->sell_off, m->sell_d16, m->sell_val, m->sell_len, m->sell_first, (long)m->rows,
+                         slices, (long)m->cols, in, d_out);
+    } else if (v == 7 || v == 8) {
       int rc = build_sell(c, const_cast<sk_mat*>(m), true, s);
       if (rc) return rc;
       const long slices = (m->rows + 63) / 64;
@@ -885,21 +1119,30 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     // each of its load instructions touches 64 rows' cache lines.  It is selectable only.)
     // SELL-64 (7) is a candidate when its copy fits in half the free memory (built here,
     // outside the timed region).
+    // Its 16-bit-step form (9) likewise, when every step fits.
     {
       int rc = build_sell(c, const_cast<sk_mat*>(m), false, s);
+      if (!rc && m->sell_col) rc = build_d16(c, const_cast<sk_mat*>(m), false, s);
       if (rc) return rc;
     }
     const bool sell = m->sell_col != nullptr;
+    const bool d16 = m->sell_d16 != nullptr;
     // (8 loads in flight per lane instead of 4 measured no faster: 45.8 vs 43.9 ms for A,
     // 63.5 vs 62.6 ms for B at 2048^2.)
-    int cand[6] = {0, 2, 3, 4, 7, 8};
+    int cand[7] = {0, 2, 3, 4};
     int NV = fpT ? 4 : 2;
     if (sell) cand[NV++] = 7;
     if (sell && fpT) cand[NV++] = 8;
-    hipEvent_t e[7];
+    if (d16) cand[NV++] = 9;
+    hipEvent_t e[8];
     for (int v = 0; v <= NV; ++v) SK_HIP(c, hipEventCreate(&e[v]));
     if (fpT) {
       int rc = need_xT();  // outside the timed region of variants 3, 4 (as for every later call)
+      if (rc) return rc;
+    }
+    if (d16 && m->line_orig) {  // variant 9's permuted input: its scratch grown before timing
+      void* ws = nullptr;
+      int rc = scratch(c, sizeof(float) * (size_t)batch * m->cols, s, &ws);
       if (rc) return rc;
     }
     SK_HIP(c, hipEventRecord(e[0], s));

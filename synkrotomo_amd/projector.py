@@ -320,8 +320,9 @@ class Matrix:
         """0: wave per row, 1: lane per row (lockstep), 2: wave per row with 16-byte streams,
         3 / 4: as 0 / 2 with the steep rays' gathers from x^T (A only), 7: lane per row on a
         SELL-64 copy of the matrix (built on first use, about the CSR's size), 8: as 7 with the
-        flat rays' gathers from x^T (A only); -1: time the candidates on next use and keep the
-        fastest."""
+        flat rays' gathers from x^T (A only), 9: as 7 with 16-bit column steps (6 bytes per
+        entry; ValueError on use when a row's consecutive columns are more than 32767 apart);
+        -1: time the candidates on next use and keep the fastest."""
         nat.check(nat.lib().sk_mat_set_spmv_variant(self.handle, int(variant)), self.geom.ctx.handle)
 
     def spmv(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

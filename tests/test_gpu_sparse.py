@@ -50,7 +50,7 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
     x = rng.random(N * N).astype(np.float32)
     y = rng.random(ang.size * R).astype(np.float32)
     geo = ctx.geometry(ang, rz, dr, R, N)
-    for variant in (0, 1, 2):
+    for variant in (0, 1, 2, 9):
         for kind in ("fp", "bp"):
             geo.sparse(kind).set_spmv_variant(variant)
         ax = geo.sparse("fp").spmv(_dev(x)).cpu().numpy()
@@ -64,25 +64,54 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
 def test_spmv_variants_bitwise(ctx, name, N, ang, R, rz, dr):
     """Every SpMV kernel that sums a row in the same order gives the same bits, batched too:
     wave per row 0 == 3 (A: steep rays gather from x^T); 16-byte streams 2 == 4 (x^T); lane
-    per row 1 == 7 (SELL-64 copy) == 8 (A: SELL-64, flat rays gather from x^T).  B rejects 3,
-    4 and 8; 5 and 6 do not exist."""
+    per row 1 == 7 (SELL-64 copy) == 8 (A: SELL-64, flat rays gather from x^T) == 9 (SELL-64
+    with 16-bit column steps).  B rejects 3, 4 and 8; 5, 6 and 10 do not exist."""
     rng = np.random.default_rng(13)
     geo = ctx.geometry(ang, rz, dr, R, N)
     for kind, ncols in (("fp", N * N), ("bp", ang.size * R)):
         x = _dev(rng.random((3, ncols)).astype(np.float32))
         M = geo.sparse(kind)
-        variants = (0, 1, 2, 3, 4, 7, 8) if kind == "fp" else (0, 1, 2, 7)
+        variants = (0, 1, 2, 3, 4, 7, 8, 9) if kind == "fp" else (0, 1, 2, 7, 9)
         out = {}
         for v in variants:
             M.set_spmv_variant(v)
             assert M.spmv_variant == v
             out[v] = M.spmv(x).cpu().numpy()
-        same = [(1, 7)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else [])
+        same = [(1, 7), (1, 9)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else [])
         for a, b in same:
             np.testing.assert_array_equal(out[b], out[a], err_msg=f"{kind}: variant {b} vs {a}")
-        for v in ((3, 4, 5, 6, 8) if kind == "bp" else (5, 6, 9)):
+        for v in ((3, 4, 5, 6, 8, 10) if kind == "bp" else (5, 6, 10)):
             with pytest.raises(ValueError):
                 M.set_spmv_variant(v)
+    geo.drop_sparse()
+
+
+def test_spmv_steps_beyond_16_bits(ctx):
+    """Variant 9 keeps each row's columns as 16-bit steps.  In B of an 8x8 image under 40000
+    bins, a pixel's entries (one or two bins per angle) lie ~40000 columns apart: asking for 9
+    fails with ValueError and the autotuner leaves it out; A of the same geometry (steps of at
+    most N + 1 pixels) runs it, bitwise equal to the lane-per-row kernel."""
+    N, R = 8, 40000
+    ang = _uniform(4)
+    geo = ctx.geometry(ang, -(R - 1) / 2.0, 1.0, R, N)
+    rng = np.random.default_rng(21)
+    y = _dev(rng.random(ang.size * R).astype(np.float32))
+    B = geo.sparse("bp")
+    B.set_spmv_variant(1)
+    ref = B.spmv(y).cpu().numpy()
+    B.set_spmv_variant(9)
+    with pytest.raises(ValueError):
+        B.spmv(y)
+    B.set_spmv_variant(-1)
+    got = B.spmv(y).cpu().numpy()
+    assert B.spmv_variant not in (-1, 9)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)  # its own summation order
+    x = _dev(rng.random(N * N).astype(np.float32))
+    A = geo.sparse("fp")
+    A.set_spmv_variant(1)
+    a1 = A.spmv(x).cpu().numpy()
+    A.set_spmv_variant(9)
+    np.testing.assert_array_equal(A.spmv(x).cpu().numpy(), a1)
     geo.drop_sparse()
 
 
@@ -110,6 +139,11 @@ def test_adj_is_transpose_of_oracle_A_bitwise(ctx, name, N, ang, R, rz, dr):
     y = np.random.default_rng(13).random(ang.size * R).astype(np.float32)
     aty = m.spmv(_dev(y)).cpu().numpy()
     assert rel_l2(aty, oracle.fp_adjoint(ang, rz, dr, R, N, y)) <= TOL
+    # lane per row on the CSR, and on the 16-bit steps over line-table order: the same bits
+    m.set_spmv_variant(1)
+    a1 = m.spmv(_dev(y)).cpu().numpy()
+    m.set_spmv_variant(9)
+    np.testing.assert_array_equal(m.spmv(_dev(y)).cpu().numpy(), a1)
     assert geo.sparse_nnz("adj") == m.nnz
     geo.drop_sparse()
 
