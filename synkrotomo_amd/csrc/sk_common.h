@@ -17,9 +17,13 @@ enum : int {
   LINE_FLAT = 1,      // flat angle: the line acts on the transposed image
   LINE_SLOPE_POS = 2, // slope >= 0
   LINE_DEGEN = 4,     // |slope| < 1e-3: x_top may round onto x_bot (theta ~ 0, pi/2)
-  LINE_FLAG_BITS = 7,
+  LINE_FLAG_BITS = 7, // the flags above (the exported line table carries these)
+  // |slope| large enough for the fast backprojector to take 1/(x_top - x_bot) by one Newton
+  // step from 1/|slope| (newton_min_slope in sk_geom.hip); set only on geometry tables
+  LINE_NEWTON = 8,
+  LINE_KEY_BITS = 15,
   // bits >= LINE_RUN_SHIFT: number of lines from this one to the end of its run of
-  // consecutive table entries with equal flag bits (>= 1)
+  // consecutive table entries with equal LINE_KEY_BITS (>= 1)
   LINE_RUN_SHIFT = 8
 };
 

@@ -47,7 +47,19 @@ int scratch(Ctx* c, size_t bytes, hipStream_t stream, void** out) {
 // Host line table.  Same operations and rounding points as sirtlib.fut:29-41:
 // cos/sin of the f32 angle, l = sqrt(1 + (a/b)**2) in f32, flat <=> |sin| >= |cos|,
 // stable partition steep-then-flat.
-static int build_lines(const float* ang, int A, std::vector<Line>& out) {
+// |slope| from which the fast backprojector takes 1/(x_top - x_bot) by one Newton step from
+// 1/|slope|.  The candidates that get a fraction strictly between 0 and 1 cross their pixel, so
+// |x_bot| <= N/2 + 4 and x_top - x_bot (exact by Sterbenz) differs from |slope| by at most half
+// an ulp of N/2 + 4.  With that relative difference eps <= 4e-4 the step leaves eps^2 <= 1.6e-7,
+// on par with v_rcp_f32's 1 ulp.  (0.15 at N = 2048.)
+static float newton_min_slope(int N) {
+  const float x = (float)(N / 2 + 4);
+  const float half_ulp = 0.5f * (nextafterf(x, INFINITY) - x);
+  return half_ulp / 4e-4f;
+}
+
+// N > 0: also mark the lines of LINE_NEWTON for an N x N image.
+static int build_lines(const float* ang, int A, std::vector<Line>& out, int N = 0) {
   const float fact = sqrtf(2.0f) / 2.0f;
   std::vector<Line> steep, flat;
   steep.reserve(A);
@@ -74,15 +86,16 @@ static int build_lines(const float* ang, int A, std::vector<Line>& out) {
     L.orig = i;
     L.flags = (is_flat ? LINE_FLAT : 0) | (L.slope >= 0.0f ? LINE_SLOPE_POS : 0) |
               (fabsf(L.slope) < 1e-3f ? LINE_DEGEN : 0);
+    if (N > 0 && !(L.flags & LINE_DEGEN) && fabsf(L.slope) >= newton_min_slope(N)) L.flags |= LINE_NEWTON;
     (is_flat ? flat : steep).push_back(L);
   }
   out = steep;
   out.insert(out.end(), flat.begin(), flat.end());
-  // run lengths (consecutive lines with equal flag bits), counted from the back
+  // run lengths (consecutive lines with equal key bits), counted from the back
   for (int j = A - 1; j >= 0; j--) {
-    const bool same = j + 1 < A && (out[j + 1].flags & LINE_FLAG_BITS) == (out[j].flags & LINE_FLAG_BITS);
+    const bool same = j + 1 < A && (out[j + 1].flags & LINE_KEY_BITS) == (out[j].flags & LINE_KEY_BITS);
     const int run = same ? (out[j + 1].flags >> LINE_RUN_SHIFT) + 1 : 1;
-    out[j].flags = (out[j].flags & LINE_FLAG_BITS) | (run << LINE_RUN_SHIFT);
+    out[j].flags = (out[j].flags & LINE_KEY_BITS) | (run << LINE_RUN_SHIFT);
   }
   return (int)steep.size();
 }
@@ -191,7 +204,7 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   G.A = A; G.R = R; G.N = N; G.rho0 = rho0; G.drho = drho; G.ncand = ncand;
   G.bp_two_cand = drho == 1.0f && ncand == 2 && std::fabs((double)rho0) + N < (double)(1 << 20);
   G.device = c->device;
-  G.n_steep = build_lines(angles_host, A, G.lines);
+  G.n_steep = build_lines(angles_host, A, G.lines, N);
   {
     // Staged forward projector: the intercepts of 256 consecutive bins span 255*drho/|cost|
     // columns, plus their rounding; a block of 16 rows shares one segment start, so the

@@ -392,7 +392,8 @@ struct alignas(16) BpLineLds {
   int w0;
   float ci, si, cst;
   float cost, lbase;  // the line's (window staging)
-  int orig, pad;
+  int orig;
+  float rs;  // 1/|slope| on LINE_NEWTON lines (fast path), else 0
 };
 
 // One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
@@ -459,7 +460,7 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
       e.cost = L.cost;
       e.lbase = L.lbase;
       e.orig = L.orig;
-      e.pad = 0;
+      e.rs = (L.flags & LINE_NEWTON) ? 1.0f / fabsf(L.slope) : 0.0f;
       lsh[threadIdx.x] = e;
     }
     __syncthreads();
@@ -516,7 +517,8 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         // lines from a line to the end of its run.  Each run is walked by a loop whose body
         // has the sign and the degenerate case resolved at compile time (no per-line branch).
         auto run = [&](auto kind, int jb, int je) {
-          constexpr int KIND = decltype(kind)::value;  // 0: slope >= 0, 1: slope < 0, 2: degenerate
+          // 0: slope >= 0, 1: slope < 0, 2: degenerate; 3, 4: as 0, 1 on LINE_NEWTON lines
+          constexpr int KIND = decltype(kind)::value;
           for (int jj = jb; jj < je; ++jj) {
             const float slope = P.lines[l0 + jj].slope;
             const BpLineLds lc = lsh[jj];
@@ -556,6 +558,16 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
                   const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
                   const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
                   acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+                } else if constexpr (KIND == 3 || KIND == 4) {
+                  // 1/d by one Newton step from 1/|slope| (d = |x_top - x_bot| is |slope| to
+                  // within the rounding of x_bot): two full-rate FMAs for the quarter-rate
+                  // v_rcp_f32
+                  const float d = KIND == 3 ? xt - xb : xb - xt;
+                  const float m = KIND == 3 ? __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f)
+                                            : __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                  const float r = fmaf(lc.rs, fmaf(-d, lc.rs, 1.0f), lc.rs);
+                  const float f = fminf(fmaxf(m * r, 0.0f), 1.0f);
+                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
                 } else {
                   // degenerate line: general form with the m == 0 case, lbase already in e.y
                   const float l = cand_len<false, W_REF>(xb, slope, 1.0f, xm[q], x1[q], true);
@@ -569,6 +581,10 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
           const int je = min(jr + (f >> LINE_RUN_SHIFT), j_end);
           if (f & LINE_DEGEN)
             run(std::integral_constant<int, 2>{}, jr, je);
+          else if ((f & LINE_NEWTON) && (f & LINE_SLOPE_POS))
+            run(std::integral_constant<int, 3>{}, jr, je);
+          else if (f & LINE_NEWTON)
+            run(std::integral_constant<int, 4>{}, jr, je);
           else if (f & LINE_SLOPE_POS)
             run(std::integral_constant<int, 0>{}, jr, je);
           else
