@@ -173,16 +173,19 @@ __device__ __forceinline__ float bcast16(float v) {
 // row crosses into it).  When the row stays in pixel Xb (the reference's full-length case)
 // the numerator has the wrong sign and the clamp to [0, 1] makes fo exactly 0, without a
 // select; a zero-length range (x/0 = inf, or 0*inf) also clamps to 0.  The sums are
-// fractions of the row length; the caller scales by lbase.
-template <bool POS>
-__device__ __forceinline__ float fp_row_add(float sum, float xb, float xt, float vb, float vo) {
+// fractions of the row length; the caller scales by lbase.  NT (LINE_NEWTON lines): the
+// reciprocal of |xtop - xbot| by one Newton step from rs = 1/|slope| (two FMAs) instead of
+// the quarter-rate v_rcp_f32 (error <= 1.6e-7 relative, see newton_min_slope).
+template <bool POS, bool NT = false>
+__device__ __forceinline__ float fp_row_add(float sum, float xb, float xt, float vb, float vo, float rs = 0.0f) {
   const float fx = floorf(xb);
+  const float d = POS ? xt - xb : xb - xt;
+  const float r = NT ? fmaf(rs, fmaf(-d, rs, 1.0f), rs) : fast_rcp(d);
   float fo;
   if constexpr (POS) {  // (xtop - (fx + 1)) / (xtop - xbot) with one rounding: xtop - fx is exact
-    const float r = fast_rcp(xt - xb);
     fo = fmaf(xt - fx, r, -r);
   } else {              // (fx - xtop) / (xbot - xtop)
-    fo = (fx - xt) * fast_rcp(xb - xt);
+    fo = (fx - xt) * r;
   }
   fo = fminf(fmaxf(fo, 0.0f), 1.0f);
   return fmaf(fo, vo - vb, sum + vb);
@@ -199,9 +202,9 @@ __device__ __forceinline__ float fp_row_add(float sum, float xb, float xt, float
 // flight while row i's fraction is formed.  Rows past i1 (the last block's tail and the
 // prefetches) read through an empty descriptor: no memory access, both values 0, and such
 // a row adds exactly 0.
-template <bool POS, int RPT>
+template <bool POS, int RPT, bool NT = false>
 __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int i0, int i1,
-                                             float slope, const float* icpt, float* sum) {
+                                             float slope, const float* icpt, float* sum, float rs = 0.0f) {
   if (i0 >= i1) return;
   constexpr int D = 3;  // rows in flight
   const int l16 = (int)(threadIdx.x & 15);
@@ -225,8 +228,8 @@ __device__ __forceinline__ void fp_rows_fast(const float* src, int N, int h, int
   auto consume = [&](int q) {
 #pragma unroll
     for (int k = 0; k < RPT; ++k)
-      sum[k] = POS ? fp_row_add<true>(sum[k], xb[q][k], xt[q][k], va[q][k], vc[q][k])
-                   : fp_row_add<false>(sum[k], xb[q][k], xt[q][k], vc[q][k], va[q][k]);
+      sum[k] = POS ? fp_row_add<true, NT>(sum[k], xb[q][k], xt[q][k], va[q][k], vc[q][k], rs)
+                   : fp_row_add<false, NT>(sum[k], xb[q][k], xt[q][k], vc[q][k], va[q][k], rs);
   };
   auto stage = [&](auto kc, int i, float tab) {
     constexpr int K = decltype(kc)::value;
@@ -296,8 +299,14 @@ __device__ __forceinline__ void fp_item(const FpParams& P, int j, int wb, int bz
 
   constexpr bool FASTW = !(WEIGHTS == W_ADJ || EXACT || MODE == FP_ONES_INV);
   if constexpr (FASTW) {
-    if (L.flags & LINE_SLOPE_POS)
+    const bool nt = L.flags & LINE_NEWTON;
+    const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
+    if ((L.flags & LINE_SLOPE_POS) && nt)
+      fp_rows_fast<true, RPT, true>(src, N, h, i0, i1, slope, icpt, sum, rs);
+    else if (L.flags & LINE_SLOPE_POS)
       fp_rows_fast<true, RPT>(src, N, h, i0, i1, slope, icpt, sum);
+    else if (nt)
+      fp_rows_fast<false, RPT, true>(src, N, h, i0, i1, slope, icpt, sum, rs);
     else
       fp_rows_fast<false, RPT>(src, N, h, i0, i1, slope, icpt, sum);
   } else {
@@ -754,10 +763,10 @@ __device__ __forceinline__ int bcast16i(int v) {
   return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 
-template <bool POS, int NBUF>
+template <bool POS, int NBUF, bool NT = false>
 __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
                                           int i0w, int i1w, float slope, float icpt,
-                                          float icpt_lo, float* seg) {
+                                          float icpt_lo, float* seg, float rs = 0.0f) {
   const int tid = threadIdx.x;
   const int l16 = tid & 15;
   const int lane = tid & 63;
@@ -826,7 +835,8 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   };
   auto sum_row = [&](int S) {
     const float xb = xbs[S], xt = xb + slope;
-    acc = POS ? fp_row_add<true>(acc, xb, xt, vas[S], vcs[S]) : fp_row_add<false>(acc, xb, xt, vcs[S], vas[S]);
+    acc = POS ? fp_row_add<true, NT>(acc, xb, xt, vas[S], vcs[S], rs)
+              : fp_row_add<false, NT>(acc, xb, xt, vcs[S], vas[S], rs);
   };
   auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + 8
     constexpr int K = decltype(kc)::value;
@@ -923,8 +933,14 @@ __device__ __forceinline__ void fps_item(const FpParams& P, int j, int blk, int 
   const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
   float acc = 0.0f;
   if (i0 < i1) {
-    if (L.flags & LINE_SLOPE_POS)
+    const bool nt = L.flags & LINE_NEWTON;
+    const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
+    if ((L.flags & LINE_SLOPE_POS) && nt)
+      acc = fps_walk<true, NBUF, true>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg, rs);
+    else if (L.flags & LINE_SLOPE_POS)
       acc = fps_walk<true, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+    else if (nt)
+      acc = fps_walk<false, NBUF, true>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg, rs);
     else
       acc = fps_walk<false, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
   }
