@@ -22,6 +22,15 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
+def oracle_threads() -> int:
+    """Threads for the oracle's OpenMP loops: OMP_NUM_THREADS when set (a host's CPU share),
+    else every core in this process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
 def load_golden(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
