@@ -135,7 +135,10 @@ typedef struct {
  * SIRT over slices of one geometry): advance every problem of every group `iters` SIRT
  * iterations.  Each iteration is two persistent launches -- one work queue over all
  * problems' forward-projection items, one over all their backprojection tiles -- and the
- * whole sequence runs as one hipGraph.  Synchronizes `stream` before returning. */
+ * whole sequence runs as one hipGraph.  The executable graph is kept (the context holds the
+ * four most recently used), keyed by the problems' geometries, batches and device pointers and
+ * the iteration count, so a repeated call replays it without capturing again.  Returns once the
+ * work is enqueued on `stream`. */
 int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iters, void* stream);
 
 /* No reference counterpart: time the kernel variants on this geometry and keep the fastest
@@ -163,13 +166,21 @@ int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, 
 /* 32 -> 1 with residual: y = res + conv(h) + bias[0], w [3][3][32]. */
 int sk_conv_out(sk_ctx* ctx, int B, int H, int W, const void* h, const void* w, const float* bias,
                 const float* res, float* y, void* stream);
+/* 2 -> 32: h = relu(conv(x2) + bias) on two FP32 input channels x2 [B][H][W][2] (the image and
+ * its SIRT update direction), w [32][3][3][2]. */
+int sk_conv_in2(sk_ctx* ctx, int B, int H, int W, const float* x2, const void* w, const float* bias,
+                void* h, void* stream);
 /* Data gradients of sk_conv_out (dh from dy) and of sk_conv_in (dx from dh masked by h > 0). */
 int sk_conv_out_dgrad(sk_ctx* ctx, int B, int H, int W, const float* g, const void* w, void* dh,
                       void* stream);
 int sk_conv_in_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
                      const void* w, float* dx, void* stream);
-/* Weight and bias gradients (FP32, deterministic): kind 0 = conv1, 1 = conv2/conv3,
- * 2 = conv4.  out = Cout*9*Cin weight gradients then Cout bias gradients. */
+/* Data gradient of sk_conv_in2: dx2 [B][H][W][2]. */
+int sk_conv_in2_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
+                      const void* w, float* dx2, void* stream);
+/* Weight and bias gradients (FP32, deterministic): kind 0 = conv1, 1 = conv2/conv3 (on MFMA),
+ * 2 = conv4, 3 = conv1 with two input channels (x = [B][H][W][2] FP32).  out = Cout*9*Cin weight
+ * gradients then Cout bias gradients. */
 int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, const void* mask,
                   const void* x, float* out, void* stream);
 /* Random-ellipse phantoms: params [B][E][7] = (cx, cy, a, b, cos t, sin t, value) in FP64 on

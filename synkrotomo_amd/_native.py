@@ -50,9 +50,11 @@ SIGNATURES = {
     "sk_geom_set_variants": (_i, [_vp, _i, _i]),
     "sk_conv32": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sk_conv_in": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_in2": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "sk_conv_out": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sk_conv_out_dgrad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "sk_conv_in_dgrad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "sk_conv_in2_dgrad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "sk_conv_wgrad": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "sk_phantoms": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "sk_sparse_build": (_i, [_vp, _vp, _i, ctypes.POINTER(_vp), _vp]),

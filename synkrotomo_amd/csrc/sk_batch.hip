@@ -281,8 +281,6 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     plan.Wmax = std::max(plan.Wmax, B.bp.W);
     plan.dr1 = plan.dr1 && G.bp_two_cand;
     args.p[oi] = B;
-    rc = launch_transpose(c, P.d_img, imgT, G.N, P.batch, s);
-    if (rc) return rc;
   }
   if (args.bp_total >= (1L << 32))
     return fail(c, SK_EINVAL, "problem set too large for one work queue");
@@ -313,11 +311,41 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     fprintf(stderr, "sk_sirt_batched: %d groups, fp mode %d items %ld (grid %d), bp 16x%d tiles items %ld (grid %d), W %d\n",
             count, plan.fp_mode, args.fp_total, fp_grid, 16 * plan.bp_ppt, args.bp_total, bp_grid, plan.Wmax);
 
-  // Capture reset + iters x (fp, bp) once, launch once.
+  // Everything the graph's kernel arguments bake in: a later call with the same key replays it.
+  std::vector<unsigned long long> key;
+  key.reserve(8 + 6 * (size_t)count);
+  for (unsigned long long v : {(unsigned long long)count, (unsigned long long)iters, (unsigned long long)ex,
+                               (unsigned long long)plan.fp_mode, (unsigned long long)plan.bp_ppt,
+                               (unsigned long long)(uintptr_t)ws, (unsigned long long)fp_grid,
+                               (unsigned long long)bp_grid})
+    key.push_back(v);
+  for (int i = 0; i < count; i++) {
+    const sk_problem& P = problems[i];
+    const Geom& G = P.geom->g;
+    for (unsigned long long v : {G.uid, (unsigned long long)P.batch, (unsigned long long)(uintptr_t)P.d_img,
+                                 (unsigned long long)(uintptr_t)P.d_sino, (unsigned long long)(uintptr_t)G.d_rowinv,
+                                 (unsigned long long)(uintptr_t)G.d_colinv})
+      key.push_back(v);
+  }
+  for (auto& bg : c->batch_graphs) {
+    if (bg.exec && bg.key == key) {
+      bg.used = ++c->batch_clock;
+      SK_HIP(c, hipGraphLaunch(bg.exec, s));
+      return SK_OK;
+    }
+  }
+
+  // Capture the transposed copies, the counter reset and iters x (fp, bp) once; keep the graph.
   hipStream_t cs = c->capture_stream;
   SK_HIP(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, cs, plan.d_ctr);
-  rc = launched(c, "counter reset");
+  for (int oi = 0; oi < count && rc == SK_OK; oi++) {
+    const BatchProb& B = args.p[oi];
+    rc = launch_transpose(c, B.fp.img, const_cast<float*>(B.fp.imgT), B.fp.N, B.batch, cs);
+  }
+  if (rc == SK_OK) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, cs, plan.d_ctr);
+    rc = launched(c, "counter reset");
+  }
   for (int it = 0; it < iters && rc == SK_OK; it++) {
     if (plan.dr1)
       rc = ex ? launch_iteration<true, 2, true>(c, plan, fp_grid, bp_grid, cs)
@@ -337,12 +365,27 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
-  e = hipGraphLaunch(exec, s);
-  // The executable graph is released after the work it enqueued has finished.
-  hipError_t e2 = hipStreamSynchronize(s);
-  (void)hipGraphExecDestroy(exec);
-  if (e != hipSuccess) return hip_fail(c, e, "hipGraphLaunch");
-  if (e2 != hipSuccess) return hip_fail(c, e2, "batched SIRT");
+  Ctx::BatchGraph* slot = nullptr;  // an empty slot, else the least recently used
+  for (auto& bg : c->batch_graphs) {
+    if (!bg.exec) {
+      slot = &bg;
+      break;
+    }
+    if (!slot || bg.used < slot->used) slot = &bg;
+  }
+  if (slot->exec) {
+    // An evicted graph may still run on some stream: wait for the device before freeing it.
+    hipError_t es = hipDeviceSynchronize();
+    if (es != hipSuccess) {
+      (void)hipGraphExecDestroy(exec);
+      return hip_fail(c, es, "batched SIRT (graph eviction)");
+    }
+    (void)hipGraphExecDestroy(slot->exec);
+  }
+  slot->exec = exec;
+  slot->key = std::move(key);
+  slot->used = ++c->batch_clock;
+  SK_HIP(c, hipGraphLaunch(exec, s));
   return SK_OK;
 }
 

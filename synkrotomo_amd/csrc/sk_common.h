@@ -57,6 +57,7 @@ struct Geom {
   int fp_variant = -1;        // tuned kernel variants (-1: the context's default)
   int bp_variant = -1;
   int device = 0;
+  unsigned long long uid = 0;  // unique per geometry created in this process (graph cache keys)
   // Cached SIRT iteration graphs and the arguments each was captured with (a few, so that
   // callers alternating between buffer sets -- double-buffered streaming -- reuse them).
   struct SirtGraph {
@@ -82,6 +83,16 @@ struct Ctx {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   hipStream_t capture_stream = nullptr;  // private stream used for hipGraph capture
+  // Executable graphs of sk_sirt_batched, keyed by every value their kernel arguments bake in
+  // (problem pointers, geometry ids, weights, scratch address, iterations, mode; LRU).
+  struct BatchGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned long long> key;
+    unsigned long long used = 0;
+  };
+  static constexpr int BATCH_GRAPHS = 4;
+  BatchGraph batch_graphs[BATCH_GRAPHS];
+  unsigned long long batch_clock = 0;
 };
 
 // Error helpers: record a message in the context and return the code.

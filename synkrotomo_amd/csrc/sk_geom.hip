@@ -6,6 +6,7 @@
 // index the sinogram through Line::orig, so neither the sinogram partition
 // (fix_projections, sirtlib.fut:16-21) nor the output scatter (postprocess_fp,
 // sirtlib.fut:24-25) exists.
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -49,9 +50,12 @@ int scratch(Ctx* c, size_t bytes, hipStream_t stream, void** out) {
 // stable partition steep-then-flat.
 // |slope| from which the fast backprojector takes 1/(x_top - x_bot) by one Newton step from
 // 1/|slope|.  The candidates that get a fraction strictly between 0 and 1 cross their pixel, so
-// |x_bot| <= N/2 + 4 and x_top - x_bot (exact by Sterbenz) differs from |slope| by at most half
-// an ulp of N/2 + 4.  With that relative difference eps <= 4e-4 the step leaves eps^2 <= 1.6e-7,
-// on par with v_rcp_f32's 1 ulp.  (0.15 at N = 2048.)
+// |x_bot|, |x_top| <= N/2 + 4.  x_top = fl(x_bot + slope) is off slope + x_bot by at most half an
+// ulp of N/2 + 4, and the difference fl(x_top - x_bot) adds a relative rounding of at most 2^-24
+// (none when x_top and x_bot are within a factor 2 of each other, by Sterbenz; otherwise it
+// rounds once).  So d = |x_top - x_bot| differs from |slope| by a relative
+// eps <= ulp(N/2 + 4)/2/|slope| + 2^-24.  With eps <= 4e-4 the step leaves eps^2 <= 1.6e-7, on par
+// with v_rcp_f32's 1 ulp.  (0.15 at N = 2048.)
 static float newton_min_slope(int N) {
   const float x = (float)(N / 2 + 4);
   const float half_ulp = 0.5f * (nextafterf(x, INFINITY) - x);
@@ -135,10 +139,10 @@ int sk_create(int device, sk_ctx** out) {
 void sk_destroy(sk_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->c.device);
-  if (ctx->c.scratch) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(ctx->c.scratch);
-  }
+  (void)hipDeviceSynchronize();  // work of this context may still use its graphs and scratch
+  for (auto& bg : ctx->c.batch_graphs)
+    if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
+  if (ctx->c.scratch) (void)hipFree(ctx->c.scratch);
   if (ctx->c.capture_stream) (void)hipStreamDestroy(ctx->c.capture_stream);
   delete ctx;
 }
@@ -199,7 +203,9 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   const int ncand = (int)ceilf(sqrtf(2.0f) / drho);  // backprojection.fut:87
   if (ncand > 16) return fail(c, SK_EINVAL, "deltarho %g too small (more than 16 rays per pixel)", drho);
   SK_HIP(c, hipSetDevice(c->device));
+  static std::atomic<unsigned long long> next_uid{1};
   sk_geom* g = new sk_geom();
+  g->g.uid = next_uid++;
   Geom& G = g->g;
   G.A = A; G.R = R; G.N = N; G.rho0 = rho0; G.drho = drho; G.ncand = ncand;
   G.bp_two_cand = drho == 1.0f && ncand == 2 && std::fabs((double)rho0) + N < (double)(1 << 20);

@@ -14,13 +14,17 @@
 //                 weights, staged once per block.  The same kernel computes the data
 //                 gradient (weights flipped and transposed on the host side), with the ReLU
 //                 derivative of the layer's output fused into the staging of its input.
-//   k_conv_in     1 -> 32 (288 MACs per pixel) on VALU with fused bias and ReLU, in FP32 on the
-//                 FP32 image; the data gradients of both thin layers likewise.
+//   k_conv_in     1 or 2 -> 32 (288 MACs per pixel and input channel) on VALU with fused bias and
+//                 ReLU, in FP32 on the FP32 input (the image, or the image and the SIRT update
+//                 direction interleaved); the data gradients of both thin layers likewise.
 //   k_conv_out_mfma  32 -> 1: per 16x16 tile, the 9 tap products of every halo pixel in one
 //                 MFMA per 16 pixels, then a 9-term shifted sum per output, fused bias and
 //                 residual.
-//   k_wgrad       weight and bias gradients: per-block partial sums over pixel tiles
-//                 (deterministic), then k_reduce_partials.
+//   k_wgrad32_mfma  weight and bias gradients of the 32 -> 32 layers on MFMA (K = pixels),
+//                 channel-major operands staged in LDS; per-block partial sums (deterministic),
+//                 then k_reduce_partials.
+//   k_wgrad       weight and bias gradients of the thin layers (1 -> 32, 32 -> 1) on VALU, the
+//                 same per-block partials.
 //   k_phantom     random-ellipse phantoms, evaluated in FP64 exactly as the numpy
 //                 generator (synkrotomo_amd/phantom.py) does.
 #include <algorithm>
@@ -146,14 +150,17 @@ __global__ __launch_bounds__(256) void k_conv32(int H, int W, long ntiles, const
   }
 }
 
-// ---------------------------------------------------------------------- 1 -> 32, VALU
-// h[p][co] = relu(b[co] + sum_t w[co][t] x[p + off_t]); w: [32][9] bf16
-// Grid (pixel blocks of one image, images).  The weights are read as BF16 pairs through
-// readfirstlane (wave-uniform values: scalar loads into SGPRs, unpacked on the scalar unit,
-// operands of the FMAs): no LDS reads for them and no per-lane conversion.  Same operations in
-// the same order as a plain per-pixel loop: bias, then taps 0..8.  The block's 256 x 32 BF16
-// outputs (16 KB, contiguous in NHWC) go out through LDS so that every store instruction
-// writes 1 KB of consecutive bytes.  w2: the [32][9] BF16 weights as 144 words.
+// ---------------------------------------------------------------------- CIN -> 32, VALU
+// h[p][co] = relu(b[co] + sum_t sum_c w[co][t][c] x[p + off_t][c]); x: FP32 [B][H][W][CIN]
+// (CIN = 1: an image; CIN = 2: the image and the SIRT update direction, interleaved), w: BF16
+// [32][9][CIN].  Grid (pixel blocks of one image, images).  The weights are read as BF16 pairs
+// through readfirstlane (wave-uniform values: scalar loads into SGPRs, unpacked on the scalar
+// unit, operands of the FMAs): no LDS reads for them and no per-lane conversion.  Same
+// operations in the same order as a plain per-pixel loop: bias, then taps 0..8 (channels in
+// order within a tap).  The block's 256 x 32 BF16 outputs (16 KB, contiguous in NHWC) go out
+// through LDS so that every store instruction writes 1 KB of consecutive bytes.  w2: the
+// weights as 144 * CIN words.
+template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_in(int H, int W, const float* __restrict__ x,
                                                  const uint32_t* __restrict__ w2,
                                                  const float* __restrict__ bias,
@@ -164,11 +171,13 @@ __global__ __launch_bounds__(256) void k_conv_in(int H, int W, const float* __re
   const bool live = r < H * W;
   const size_t img = (size_t)blockIdx.y * H * W;
   const int py = r / W, px = r - py * W;
-  float v[9];
+  float v[9][CIN];
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-    v[t] = (live && yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[img + (size_t)yy * W + xx] : 0.0f;
+    const bool in = live && yy >= 0 && yy < H && xx >= 0 && xx < W;
+#pragma unroll
+    for (int c = 0; c < CIN; ++c) v[t][c] = in ? x[(img + (size_t)yy * W + xx) * CIN + c] : 0.0f;
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -178,11 +187,13 @@ __global__ __launch_bounds__(256) void k_conv_in(int H, int W, const float* __re
       const int co = q * 8 + i;
       float s = bias[co];
 #pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        const int k = co * 9 + t;
-        const uint32_t pr = __builtin_amdgcn_readfirstlane(w2[k >> 1]);
-        s = fmaf(__builtin_bit_cast(float, (k & 1) ? (pr & 0xffff0000u) : (pr << 16)), v[t], s);
-      }
+      for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CIN; ++c) {
+          const int k = (co * 9 + t) * CIN + c;
+          const uint32_t pr = __builtin_amdgcn_readfirstlane(w2[k >> 1]);
+          s = fmaf(__builtin_bit_cast(float, (k & 1) ? (pr & 0xffff0000u) : (pr << 16)), v[t][c], s);
+        }
       o[i] = (__bf16)fmaxf(s, 0.0f);
     }
     st[threadIdx.x * 4 + q] = o;
@@ -273,20 +284,24 @@ __global__ __launch_bounds__(256) void k_conv_out_dgrad(int H, int W, long npix,
   }
 }
 
-// dx[q] = sum_{t,co} w[co][t] g1[q - off_t][co],  g1 = dh * (h > 0)   (data gradient of k_conv_in)
+// dx[q][c] = sum_{t,co} w[co][t][c] g1[q - off_t][co],  g1 = dh * (h > 0)   (data gradient of
+// k_conv_in; dx FP32 [B][H][W][CIN])
+template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_in_dgrad(int H, int W, long npix, const __bf16* __restrict__ dh,
                                                        const __bf16* __restrict__ hmask,
                                                        const __bf16* __restrict__ w,
                                                        float* __restrict__ dx) {
-  __shared__ float sw[NCH * 9];
-  for (int i = threadIdx.x; i < NCH * 9; i += 256) sw[i] = (float)w[i];
+  __shared__ float sw[NCH * 9 * CIN];
+  for (int i = threadIdx.x; i < NCH * 9 * CIN; i += 256) sw[i] = (float)w[i];
   __syncthreads();
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= npix) return;
   const long b = p / ((long)H * W);
   const int r = (int)(p - b * H * W);
   const int py = r / W, px = r - py * W;
-  float s = 0.0f;
+  float s[CIN];
+#pragma unroll
+  for (int c = 0; c < CIN; ++c) s[c] = 0.0f;
   for (int t = 0; t < 9; ++t) {
     const int yy = py - (t / 3 - 1), xx = px - (t % 3 - 1);
     if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
@@ -295,17 +310,20 @@ __global__ __launch_bounds__(256) void k_conv_in_dgrad(int H, int W, long npix, 
       const bf16x8 gv = *(const bf16x8*)(dh + off + q * 8);
       const bf16x8 mv = *(const bf16x8*)(hmask + off + q * 8);
       for (int i = 0; i < 8; ++i)
-        if ((float)mv[i] > 0.0f) s = fmaf(sw[(q * 8 + i) * 9 + t], (float)gv[i], s);
+        if ((float)mv[i] > 0.0f)
+#pragma unroll
+          for (int c = 0; c < CIN; ++c) s[c] = fmaf(sw[((q * 8 + i) * 9 + t) * CIN + c], (float)gv[i], s[c]);
     }
   }
-  dx[p] = s;
+#pragma unroll
+  for (int c = 0; c < CIN; ++c) dx[p * CIN + c] = s[c];
 }
 
 // ---------------------------------------------------------------------- weight gradients
 // dW[co][t][ci] = sum_p G[p][co] X[p + off_t][ci],  db[co] = sum_p G[p][co]
 // G: CO channels (bf16 NHWC, optionally masked by M > 0) or an FP32 image (CO == 1).
-// X: CI channels (bf16 NHWC) or an FP32 image (CI == 1).  Each block accumulates over a
-// grid-stride set of 16x16 tiles and writes CO*9*CI + CO partial sums.
+// X: CI = 32 channels (bf16 NHWC) or CI <= 2 FP32 channels ([B][H][W][CI]).  Each block
+// accumulates over a grid-stride set of 16x16 tiles and writes CO*9*CI + CO partial sums.
 template <int CO, int CI>
 __global__ __launch_bounds__(256) void k_wgrad(int H, int W, int B, const void* __restrict__ Gp,
                                                const __bf16* __restrict__ M,
@@ -315,7 +333,7 @@ __global__ __launch_bounds__(256) void k_wgrad(int H, int W, int B, const void* 
   constexpr int PER = (NOUT + 255) / 256;
   // 32-channel operands are BF16 already and stay BF16 in LDS; FP32 images stay FP32.
   using GT = typename std::conditional<CO == 1, float, __bf16>::type;
-  using XT = typename std::conditional<CI == 1, float, __bf16>::type;
+  using XT = typename std::conditional<CI <= 2, float, __bf16>::type;
   __shared__ GT sg[CT * CT * CO];
   __shared__ XT sx[(CT + 2) * (CT + 2) * CI];
   float acc[PER];
@@ -348,8 +366,8 @@ __global__ __launch_bounds__(256) void k_wgrad(int H, int W, int B, const void* 
       float v = 0.0f;
       if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
         const size_t p = ((size_t)b * H + gy) * W + gx;
-        if constexpr (CI == 1)
-          v = ((const float*)Xp)[p];
+        if constexpr (CI <= 2)
+          v = ((const float*)Xp)[p * CI + ci];
         else
           v = (float)((const __bf16*)Xp)[p * CI + ci];
       }
@@ -379,6 +397,137 @@ __global__ __launch_bounds__(256) void k_wgrad(int H, int W, int B, const void* 
     const int o = threadIdx.x + 256 * k;
     if (o < NOUT) partials[(size_t)blockIdx.x * NOUT + o] = acc[k];
   }
+}
+
+// 32 -> 32 weight and bias gradients on MFMA: for each tap t, dW[co][t][ci] is a GEMM with M = 16
+// output channels, N = 16 input channels and K = pixels (v_mfma_f32_16x16x32_bf16, K = 32 pixels
+// per step); db[co] is the same GEMM against a column of ones.  The MFMA takes 8 consecutive K
+// values per lane from one row of A (and of B), so the NHWC operands are staged in LDS channel-
+// major: gT[co][pixel] for the 16x16 tile's gradient (masked by M > 0), and three copies of the
+// 18x18 input halo, xs[kx][ci][halo row][16 columns] with column j holding halo column j + kx,
+// so that a tap's shifted rows start 16-byte aligned.  A K step covers two pixel rows of the
+// tile (lane group kq: row 2s + kq/2, columns 8(kq%2) .. +7); wave w takes K steps 2w and 2w+1 of
+// every tile of its block and keeps all 36 tap x channel-block accumulators.  At the end the four
+// waves' sums are added in LDS in wave order and the block writes one partial (deterministic).
+constexpr int WG_GS = 256 + 8;       // gT row (elements): 256 pixels + pad (528 B, 16 B multiple)
+constexpr int WG_XS = 18 * 16 + 8;   // xs channel row: 18 halo rows x 16 columns + pad (592 B)
+constexpr int WG_NOUT = NCH * 9 * NCH + NCH;
+
+__global__ __launch_bounds__(256) void k_wgrad32_mfma(int H, int W, int B, const __bf16* __restrict__ G,
+                                                      const __bf16* __restrict__ M,
+                                                      const __bf16* __restrict__ X,
+                                                      float* __restrict__ partials) {
+  __shared__ __attribute__((aligned(16))) __bf16 gT[NCH * WG_GS];
+  __shared__ __attribute__((aligned(16))) __bf16 xs[3 * NCH * WG_XS];
+  static_assert(sizeof(__bf16) * 3 * NCH * WG_XS >= sizeof(float) * WG_NOUT, "reduction buffer");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  f32x4 acc[9][2][2];
+  f32x4 accb[2];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[t][a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  accb[0] = accb[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x8 ones;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) ones[i] = (__bf16)1.0f;
+  const int tiles_x = (W + CT - 1) / CT, tiles_y = (H + CT - 1) / CT;
+  const long ntiles = (long)tiles_x * tiles_y * B;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {  // uniform over the block
+    const long b = tile / ((long)tiles_x * tiles_y);
+    const int tr = (int)(tile - b * tiles_x * tiles_y);
+    const int ty0 = (tr / tiles_x) * CT, tx0 = (tr % tiles_x) * CT;
+    const size_t img = (size_t)b * H * W;
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (int c = threadIdx.x; c < CT * CT * 4; c += 256) {  // gradient: 256 pixels x 4 quads
+      const int pix = c >> 2, q = c & 3;
+      const int gy = ty0 + (pix >> 4), gx = tx0 + (pix & 15);
+      bf16x8 v;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
+      if (gy < H && gx < W) {
+        const size_t off = (img + (size_t)gy * W + gx) * NCH + 8 * q;
+        v = *(const bf16x8*)(G + off);
+        if (M) {
+          const bf16x8 mk = *(const bf16x8*)(M + off);
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (!((float)mk[i] > 0.0f)) v[i] = (__bf16)0.0f;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) gT[(8 * q + i) * WG_GS + pix] = v[i];
+    }
+    for (int c = threadIdx.x; c < (CT + 2) * (CT + 2) * 4; c += 256) {  // input halo: 324 pixels x 4 quads
+      const int pix = c >> 2, q = c & 3;
+      const int hy = pix / (CT + 2), hx = pix - hy * (CT + 2);
+      const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
+      bf16x8 v;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
+      if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = *(const bf16x8*)(X + (img + (size_t)gy * W + gx) * NCH + 8 * q);
+#pragma unroll
+      for (int kx = 0; kx < 3; ++kx) {
+        const int j = hx - kx;
+        if (j < 0 || j >= CT) continue;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) xs[(kx * NCH + 8 * q + i) * WG_XS + hy * CT + j] = v[i];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ss = 0; ss < 2; ++ss) {
+      const int py = 2 * (2 * wv + ss) + (kq >> 1), px0 = (kq & 1) * 8;
+      // A: gradient of output channels cb*16 + m at the 8 pixels (py, px0 .. px0 + 7)
+      const bf16x8 a0 = *(const bf16x8*)(gT + m * WG_GS + py * CT + px0);
+      const bf16x8 a1 = *(const bf16x8*)(gT + (16 + m) * WG_GS + py * CT + px0);
+      accb[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, ones, accb[0], 0, 0, 0);
+      accb[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, ones, accb[1], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int ky = t / 3, kx = t - 3 * (t / 3);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+          // B: input channel nb*16 + m at the same pixels shifted by the tap
+          const bf16x8 bv = *(const bf16x8*)(xs + (kx * NCH + nb * 16 + m) * WG_XS + (py + ky) * CT + px0);
+          acc[t][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[t][0][nb], 0, 0, 0);
+          acc[t][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[t][1][nb], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // C: lane (m, kq) holds rows 4kq + i (output channels cb*16 + 4kq + i) of column m (input
+  // channel nb*16 + m).  Waves add their sums in order 0, 1, 2, 3.
+  float* red = (float*)xs;
+  for (int w = 0; w < 4; ++w) {
+    __syncthreads();
+    if (wv == w) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int o = ((cb * 16 + 4 * kq + i) * 9 + t) * NCH + nb * 16 + m;
+              red[o] = (w ? red[o] : 0.0f) + acc[t][cb][nb][i];
+            }
+      if (m == 0)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int o = NCH * 9 * NCH + cb * 16 + 4 * kq + i;
+            red[o] = (w ? red[o] : 0.0f) + accb[cb][i];
+          }
+    }
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < WG_NOUT; o += 256) partials[(size_t)blockIdx.x * WG_NOUT + o] = red[o];
 }
 
 __global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partials, int nblocks,
@@ -457,16 +606,31 @@ int sk_conv32(sk_ctx* ctx, int B, int H, int W, const void* x, const void* mask,
   return launched(&ctx->c, "conv32 kernel");
 }
 
-int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, const float* bias,
-               void* h, void* stream) {
+static int conv_in_go(sk_ctx* ctx, int cin, int B, int H, int W, const float* x, const void* w, const float* bias,
+                      void* h, void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (B < 1 || H < 1 || W < 1 || !x || !w || !bias || !h) return fail(&ctx->c, SK_EINVAL, "conv_in: bad arguments");
   if (B > 65535 || (long)H * W >= (1L << 31)) return fail(&ctx->c, SK_EINVAL, "conv_in: batch or image too large");
   if ((uintptr_t)w % 4) return fail(&ctx->c, SK_EINVAL, "conv_in: weights must be 4-byte aligned");
-  hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((H * W + 255) / 256), B), dim3(256), 0, (hipStream_t)stream, H, W,
-                     x, (const uint32_t*)w, bias, (__bf16*)h);
+  const dim3 grid((unsigned)((H * W + 255) / 256), B);
+  if (cin == 1)
+    hipLaunchKernelGGL(k_conv_in<1>, grid, dim3(256), 0, (hipStream_t)stream, H, W, x, (const uint32_t*)w, bias,
+                       (__bf16*)h);
+  else
+    hipLaunchKernelGGL(k_conv_in<2>, grid, dim3(256), 0, (hipStream_t)stream, H, W, x, (const uint32_t*)w, bias,
+                       (__bf16*)h);
   return launched(&ctx->c, "conv_in kernel");
+}
+
+int sk_conv_in(sk_ctx* ctx, int B, int H, int W, const float* x, const void* w, const float* bias,
+               void* h, void* stream) {
+  return conv_in_go(ctx, 1, B, H, W, x, w, bias, h, stream);
+}
+
+int sk_conv_in2(sk_ctx* ctx, int B, int H, int W, const float* x2, const void* w, const float* bias,
+                void* h, void* stream) {
+  return conv_in_go(ctx, 2, B, H, W, x2, w, bias, h, stream);
 }
 
 int sk_conv_out(sk_ctx* ctx, int B, int H, int W, const void* h, const void* w, const float* bias,
@@ -493,33 +657,62 @@ int sk_conv_out_dgrad(sk_ctx* ctx, int B, int H, int W, const float* g, const vo
   return launched(&ctx->c, "conv_out_dgrad kernel");
 }
 
-int sk_conv_in_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
-                     const void* w, float* dx, void* stream) {
+static int conv_in_dgrad_go(sk_ctx* ctx, int cin, int B, int H, int W, const void* dh, const void* hmask,
+                            const void* w, float* dx, void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (B < 1 || H < 1 || W < 1 || !dh || !hmask || !w || !dx)
     return fail(&ctx->c, SK_EINVAL, "conv_in_dgrad: bad arguments");
   const long npix = (long)B * H * W;
-  hipLaunchKernelGGL(k_conv_in_dgrad, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                     H, W, npix, (const __bf16*)dh, (const __bf16*)hmask, (const __bf16*)w, dx);
+  const dim3 grid((unsigned)((npix + 255) / 256));
+  if (cin == 1)
+    hipLaunchKernelGGL(k_conv_in_dgrad<1>, grid, dim3(256), 0, (hipStream_t)stream, H, W, npix, (const __bf16*)dh,
+                       (const __bf16*)hmask, (const __bf16*)w, dx);
+  else
+    hipLaunchKernelGGL(k_conv_in_dgrad<2>, grid, dim3(256), 0, (hipStream_t)stream, H, W, npix, (const __bf16*)dh,
+                       (const __bf16*)hmask, (const __bf16*)w, dx);
   return launched(&ctx->c, "conv_in_dgrad kernel");
+}
+
+int sk_conv_in_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
+                     const void* w, float* dx, void* stream) {
+  return conv_in_dgrad_go(ctx, 1, B, H, W, dh, hmask, w, dx, stream);
+}
+
+int sk_conv_in2_dgrad(sk_ctx* ctx, int B, int H, int W, const void* dh, const void* hmask,
+                      const void* w, float* dx2, void* stream) {
+  return conv_in_dgrad_go(ctx, 2, B, H, W, dh, hmask, w, dx2, stream);
 }
 
 // kind 0: conv1 (G = 32-ch gradient masked by h1, X = fp32 image) -> dW [32][9], db [32]
 // kind 1: conv2/3 (G masked, X = 32-ch activation)                  -> dW [32][9][32], db [32]
 // kind 2: conv4 (G = fp32 image gradient, X = 32-ch activation)     -> dW [9][32], db [1]
+// kind 3: conv1 with two input channels (G masked, X = fp32 [B][H][W][2]) -> dW [32][9][2], db [32]
 // out: CO*9*CI weight gradients followed by CO bias gradients (fp32).
 int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, const void* mask,
                   const void* x, float* out, void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   Ctx* c = &ctx->c;
-  if (B < 1 || H < 1 || W < 1 || !g || !x || !out || kind < 0 || kind > 2)
+  if (B < 1 || H < 1 || W < 1 || !g || !x || !out || kind < 0 || kind > 3)
     return fail(c, SK_EINVAL, "conv_wgrad: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   const long ntiles = (long)((W + CT - 1) / CT) * ((H + CT - 1) / CT) * B;
-  const int nblocks = (int)std::min<long>(ntiles, 1024);
-  const int nout = kind == 0 ? NCH * 9 + NCH : (kind == 1 ? NCH * 9 * NCH + NCH : 9 * NCH + 1);
+  long cap = 1024;
+  if (kind == 1) {  // the MFMA kernel: one resident set of blocks (queried once per device)
+    static int wg_resident[64] = {0};
+    const int dev = c->device < 64 ? c->device : 63;
+    if (!wg_resident[dev]) {
+      int cus = 0, per_cu = 0;
+      SK_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+      SK_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wgrad32_mfma, 256, 0));
+      wg_resident[dev] = std::max(1, cus * std::max(1, per_cu));
+    }
+    cap = wg_resident[dev];
+  }
+  const int nblocks = (int)std::min<long>(ntiles, cap);
+  const int nout = kind == 0 ? NCH * 9 + NCH
+                             : (kind == 1 ? NCH * 9 * NCH + NCH : (kind == 2 ? 9 * NCH + 1 : NCH * 9 * 2 + NCH));
   void* ws = nullptr;
   rc = scratch(c, sizeof(float) * (size_t)nblocks * nout, s, &ws);
   if (rc) return rc;
@@ -528,9 +721,12 @@ int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, con
   if (kind == 0)
     hipLaunchKernelGGL((k_wgrad<NCH, 1>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
   else if (kind == 1)
-    hipLaunchKernelGGL((k_wgrad<NCH, NCH>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
-  else
+    hipLaunchKernelGGL(k_wgrad32_mfma, dim3(nblocks), dim3(256), 0, s, H, W, B, (const __bf16*)g, Mk,
+                       (const __bf16*)x, part);
+  else if (kind == 2)
     hipLaunchKernelGGL((k_wgrad<1, NCH>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
+  else
+    hipLaunchKernelGGL((k_wgrad<NCH, 2>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
   rc = launched(c, "wgrad kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(k_reduce_partials, dim3((nout + 255) / 256), dim3(256), 0, s, part, nblocks, nout, out);

@@ -36,45 +36,53 @@ def _chk(rc, ctx, what):
 
 
 class _Correction(torch.autograd.Function):
+    """y = res + conv4(relu(conv3(relu(conv2(relu(conv1(inp))))))); inp is the image [B, H, W]
+    (one input channel) or the image and its SIRT update direction [B, H, W, 2]."""
+
     @staticmethod
-    def forward(fctx, x, w1, b1, w2, b2, w3, b3, w4, b4, ctx: Context):
+    def forward(fctx, inp, res, w1, b1, w2, b2, w3, b3, w4, b4, ctx: Context):
         L = nat.lib()
-        x = x.contiguous()
-        B, H, W = x.shape
-        dev = x.device
+        inp = inp.contiguous()
+        res = res.contiguous()
+        B, H, W = res.shape
+        cin = 1 if inp.dim() == 3 else inp.shape[-1]
+        dev = res.device
         s = _stream(ctx.device)
-        w1b = w1.to(BF16).contiguous()          # [32, 3, 3]
+        w1b = w1.to(BF16).contiguous()          # [32, 3, 3] or [32, 3, 3, 2]
         w2b = w2.to(BF16).contiguous()          # [32, 3, 3, 32]
         w3b = w3.to(BF16).contiguous()
         w4b = w4.to(BF16).contiguous()          # [3, 3, 32]
         h1 = torch.empty((B, H, W, NCH), dtype=BF16, device=dev)
         h2 = torch.empty_like(h1)
         h3 = torch.empty_like(h1)
-        y = torch.empty_like(x)
+        y = torch.empty_like(res)
         h = ctx.handle
-        _chk(L.sk_conv_in(h, B, H, W, _ptr(x), _ptr(w1b), _ptr(b1), _ptr(h1), s), ctx, "conv1")
+        conv_in = L.sk_conv_in if cin == 1 else L.sk_conv_in2
+        _chk(conv_in(h, B, H, W, _ptr(inp), _ptr(w1b), _ptr(b1), _ptr(h1), s), ctx, "conv1")
         _chk(L.sk_conv32(h, B, H, W, _ptr(h1), None, _ptr(w2b), _ptr(b2), _ptr(h2), s), ctx, "conv2")
         _chk(L.sk_conv32(h, B, H, W, _ptr(h2), None, _ptr(w3b), _ptr(b3), _ptr(h3), s), ctx, "conv3")
-        _chk(L.sk_conv_out(h, B, H, W, _ptr(h3), _ptr(w4b), _ptr(b4), _ptr(x), _ptr(y), s), ctx, "conv4")
+        _chk(L.sk_conv_out(h, B, H, W, _ptr(h3), _ptr(w4b), _ptr(b4), _ptr(res), _ptr(y), s), ctx, "conv4")
         fctx.ctx = ctx
-        fctx.save_for_backward(x, h1, h2, h3, w1b, w2b, w3b, w4b)
+        fctx.cin = cin
+        fctx.save_for_backward(inp, h1, h2, h3, w1b, w2b, w3b, w4b)
         return y
 
     @staticmethod
     def backward(fctx, gy):
         ctx = fctx.ctx
+        cin = fctx.cin
         L = nat.lib()
         h = ctx.handle
         s = _stream(ctx.device)
-        x, h1, h2, h3, w1b, w2b, w3b, w4b = fctx.saved_tensors
+        inp, h1, h2, h3, w1b, w2b, w3b, w4b = fctx.saved_tensors
         gy = gy.contiguous().float()
         B, H, W = gy.shape
         dev = gy.device
 
-        def wgrad(kind, g, mask, inp, n):
+        def wgrad(kind, g, mask, x, n):
             out = torch.empty(n, dtype=torch.float32, device=dev)
             _chk(L.sk_conv_wgrad(h, kind, B, H, W, _ptr(g), None if mask is None else _ptr(mask),
-                                 _ptr(inp), _ptr(out), s), ctx, "wgrad")
+                                 _ptr(x), _ptr(out), s), ctx, "wgrad")
             return out
 
         # conv4 (32 -> 1, residual)
@@ -90,29 +98,37 @@ class _Correction(torch.autograd.Function):
         w2t = w2b.flip(1, 2).permute(3, 1, 2, 0).contiguous()
         dh1 = torch.empty_like(h1)
         _chk(L.sk_conv32(h, B, H, W, _ptr(dh2), _ptr(h2), _ptr(w2t), None, _ptr(dh1), s), ctx, "conv2 dgrad")
-        # conv1 (1 -> 32)
-        g1 = wgrad(0, dh1, h1, x, NCH * 9 + NCH)
-        dx = torch.empty_like(x)
-        _chk(L.sk_conv_in_dgrad(h, B, H, W, _ptr(dh1), _ptr(h1), _ptr(w1b), _ptr(dx), s), ctx, "conv1 dgrad")
-        dx = dx + gy  # residual path
-        n1, n2, n4 = NCH * 9, NCH * 9 * NCH, 9 * NCH
-        return (dx, g1[:n1].view(NCH, 3, 3), g1[n1:], g2[:n2].view(NCH, 3, 3, NCH), g2[n2:],
+        # conv1 (cin -> 32)
+        n1 = NCH * 9 * cin
+        g1 = wgrad(0 if cin == 1 else 3, dh1, h1, inp, n1 + NCH)
+        dinp = torch.empty_like(inp)
+        dgrad = L.sk_conv_in_dgrad if cin == 1 else L.sk_conv_in2_dgrad
+        _chk(dgrad(h, B, H, W, _ptr(dh1), _ptr(h1), _ptr(w1b), _ptr(dinp), s), ctx, "conv1 dgrad")
+        n2, n4 = NCH * 9 * NCH, 9 * NCH
+        gw1 = g1[:n1].view(NCH, 3, 3) if cin == 1 else g1[:n1].view(NCH, 3, 3, cin)
+        return (dinp, gy, gw1, g1[n1:], g2[:n2].view(NCH, 3, 3, NCH), g2[n2:],
                 g3[:n2].view(NCH, 3, 3, NCH), g3[n2:], g4[:n4].view(3, 3, NCH), g4[n4:], None)
 
 
 class Correction(torch.nn.Module):
-    """The correction network; call on FP32 images [B, H, W] on the context's device."""
+    """The correction network; call on FP32 images [B, H, W] on the context's device.  It also
+    holds the per-stage relaxation factors of the SIRT steps it is interleaved with."""
 
-    def __init__(self, ctx: Context | None = None, seed: int = 0):
+    MAX_STAGES = 16
+
+    def __init__(self, ctx: Context | None = None, seed: int = 0, cin: int = 1):
         super().__init__()
+        if cin not in (1, 2):
+            raise ValueError("cin must be 1 (image) or 2 (image and SIRT update direction)")
         self.ctx = ctx or Context.default(0)
+        self.cin = cin
         g = torch.Generator().manual_seed(seed)
 
         def kaiming(shape, fan_in, gain=math.sqrt(2.0)):
             return torch.randn(shape, generator=g) * (gain / math.sqrt(fan_in))
 
         dev = torch.device("cuda", self.ctx.device)
-        self.w1 = torch.nn.Parameter(kaiming((NCH, 3, 3), 9).to(dev))
+        self.w1 = torch.nn.Parameter(kaiming((NCH, 3, 3) if cin == 1 else (NCH, 3, 3, cin), 9 * cin).to(dev))
         self.b1 = torch.nn.Parameter(torch.zeros(NCH, device=dev))
         self.w2 = torch.nn.Parameter(kaiming((NCH, 3, 3, NCH), 9 * NCH).to(dev))
         self.b2 = torch.nn.Parameter(torch.zeros(NCH, device=dev))
@@ -121,11 +137,23 @@ class Correction(torch.nn.Module):
         # the last layer starts at zero: the network begins as the identity
         self.w4 = torch.nn.Parameter(torch.zeros((3, 3, NCH), device=dev))
         self.b4 = torch.nn.Parameter(torch.zeros(1, device=dev))
+        # relaxation of the SIRT step of each stage k >= 1 (x + lam_k * (step(x) - x)); stage 0
+        # takes the plain step, which the fallback rule's plain SIRT shares
+        self.relax = torch.nn.Parameter(torch.ones(self.MAX_STAGES, device=dev))
 
-    def forward(self, x: torch.Tensor) -> torch.Tensor:
+    def forward(self, x: torch.Tensor, d: torch.Tensor | None = None) -> torch.Tensor:
+        """x: images [B, H, W] (or [H, W]); d: their SIRT update directions (two-channel
+        networks only)."""
         squeeze = x.dim() == 2
         xb = x.unsqueeze(0) if squeeze else x
-        y = _Correction.apply(xb, self.w1, self.b1, self.w2, self.b2, self.w3, self.b3, self.w4,
+        if self.cin == 2:
+            if d is None:
+                raise ValueError("a two-channel correction needs the SIRT update direction")
+            db = d.unsqueeze(0) if squeeze else d
+            inp = torch.stack([xb, db], dim=-1)
+        else:
+            inp = xb
+        y = _Correction.apply(inp, xb, self.w1, self.b1, self.w2, self.b2, self.w3, self.b3, self.w4,
                               self.b4, self.ctx)
         return y.squeeze(0) if squeeze else y
 
@@ -137,6 +165,12 @@ class Correction(torch.nn.Module):
     def load(self, path: str) -> dict:
         z = np.load(path, allow_pickle=False)
         sd = {k: torch.from_numpy(z[k]) for k in z.files if not k.startswith("meta_")}
+        if "relax" not in sd:  # weights trained without stage relaxation: plain steps
+            sd["relax"] = torch.ones(self.MAX_STAGES)
+        cin = 1 if sd["w1"].dim() == 3 else int(sd["w1"].shape[-1])
+        if cin != self.cin:  # the file decides the input channels
+            self.cin = cin
+            self.w1 = torch.nn.Parameter(torch.empty(tuple(sd["w1"].shape), device=self.w1.device))
         self.load_state_dict(sd)
         return {k[5:]: z[k] for k in z.files if k.startswith("meta_")}
 
@@ -183,14 +217,50 @@ def residual(geom: Geometry, x: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
 
 # ---------------------------------------------------------------------------- model
 class LearnedSIRT:
-    """``stages`` x (SIRT step, correction) with the plain-SIRT fallback."""
+    """``stages`` x (SIRT step, correction) with the plain-SIRT fallback.
 
-    def __init__(self, geom: Geometry, net: Correction, stages: int = 5):
-        self.geom, self.net, self.stages = geom, net, stages
+    Stage k takes the SIRT step relaxed by the network's factor lam_k (k >= 1; the first step
+    is plain), then the correction.  ``tail_`` continues a reconstruction with SIRT steps
+    relaxed by ``tail_relax`` (1 = the reference's SIRT)."""
+
+    def __init__(self, geom: Geometry, net: Correction, stages: int = 5, tail_relax: float = 1.0):
+        if not 1 <= stages <= Correction.MAX_STAGES:
+            raise ValueError(f"stages must be in [1, {Correction.MAX_STAGES}]")
+        if not 0.0 < tail_relax < 2.0:
+            raise ValueError("tail_relax must be in (0, 2)")
+        self.geom, self.net, self.stages, self.tail_relax = geom, net, stages, float(tail_relax)
         self.w = SirtWeights(geom)
 
     def sirt_step(self, x, b):
         return _SirtStep.apply(x, b, self.geom, self.w.rowinv, self.w.colinv)
+
+    def stage_step(self, x, b, k: int):
+        """The SIRT step of stage k: x + lam_k (step(x) - x), lam_0 = 1."""
+        s = self.sirt_step(x, b)
+        return s if k == 0 else x + self.net.relax[k] * (s - x)
+
+    def stage(self, x, b, k: int):
+        """Stage k: the relaxed SIRT step, then the correction (which also sees the step's
+        update direction when it has two input channels)."""
+        xs = self.stage_step(x, b, k)
+        return self.net(xs, xs - x) if self.net.cin == 2 else self.net(xs)
+
+    @torch.no_grad()
+    def tail_(self, x: torch.Tensor, b: torch.Tensor, iters: int, relax: float | None = None) -> torch.Tensor:
+        """``iters`` SIRT steps in place on x [B, N, N], each relaxed by ``relax`` (default:
+        tail_relax): x + lam C B(R (b - A x)), with lam folded into C."""
+        lam = self.tail_relax if relax is None else float(relax)
+        if iters <= 0:
+            return x
+        if lam == 1.0:
+            return self.geom.sirt_(x, b, iters)
+        key = (lam, self.w.colinv.data_ptr())
+        if getattr(self, "_tail_key", None) != key:
+            self._tail_colinv, self._tail_key = self.w.colinv * lam, key
+        q = torch.empty(b.shape, dtype=torch.float32, device=b.device)
+        for _ in range(iters):
+            self.geom.sirt_step_(x, b, self.w.rowinv, self._tail_colinv, q)
+        return x
 
     def unrolled(self, b: torch.Tensor, checkpoint: bool = True, all_stages: bool = False):
         """Differentiable forward used for training: images [B, N, N] after the last stage
@@ -199,11 +269,11 @@ class LearnedSIRT:
         B = b.shape[0]
         x = torch.zeros((B, self.geom.N, self.geom.N), device=b.device)
         outs = []
-        for _ in range(self.stages):
+        for k in range(self.stages):
             if checkpoint and torch.is_grad_enabled():
-                x = ck(lambda xx, bb: self.net(self.sirt_step(xx, bb)), x, b, use_reentrant=False)
+                x = ck(lambda xx, bb, kk=k: self.stage(xx, bb, kk), x, b, use_reentrant=False)
             else:
-                x = self.net(self.sirt_step(x, b))
+                x = self.stage(x, b, k)
             outs.append(x)
         return outs if all_stages else x
 
@@ -214,11 +284,11 @@ class LearnedSIRT:
         x = torch.zeros((B, self.geom.N, self.geom.N), device=b.device)
         xp = None
         for k in range(self.stages):
-            xs = self.sirt_step(x, b)
+            xs = self.stage_step(x, b, k)
             # both paths start from zero, so plain SIRT's first step is the first stage's (the
             # kernels are deterministic: the same bits)
             xp = xs.clone() if k == 0 else self.sirt_step(xp, b)
-            xc = self.net(xs)
+            xc = self.net(xs, xs - x) if self.net.cin == 2 else self.net(xs)
             bad = ~torch.isfinite(xc).reshape(B, -1).all(dim=1)
             x = torch.where(bad.view(B, 1, 1), xs, xc)
         worse = residual(self.geom, x, b) > residual(self.geom, xp, b)

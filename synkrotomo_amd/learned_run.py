@@ -38,7 +38,7 @@ from .learned import Correction, LearnedSIRT, gpu_phantoms, noisy_sinogram, resi
 from .projector import Context, centred_bins, uniform_angles
 
 N, A, STAGES, I0, ELLIPSES = 512, 360, 5, 1e4, 10
-TRAIN_SEED, TEST_SEED, NOISE_SEED = 2025, 777, 99
+TRAIN_SEED, TEST_SEED, NOISE_SEED, VALID_SEED = 2025, 777, 99, 555
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "learned_sirt_512_360.npz")
 
@@ -66,7 +66,7 @@ def train(args) -> int:
     rng = np.random.default_rng(TRAIN_SEED)
     scale = attenuation_scale(ctx, geo, np.random.default_rng(TRAIN_SEED + 1))
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED)
-    net = Correction(ctx, seed=0)
+    net = Correction(ctx, seed=0, cin=args.cin)
     if args.init:
         net.load(args.init)  # fine-tune from earlier weights
     model = LearnedSIRT(geo, net, args.stages)
@@ -144,47 +144,85 @@ def train(args) -> int:
     return 0
 
 
-def _first_hit(res: np.ndarray, target: np.ndarray) -> np.ndarray:
-    """res [steps, B] -> first step index (1-based) with res <= target, or -1."""
+def _first_hit(ks: np.ndarray, res: np.ndarray, target: np.ndarray) -> np.ndarray:
+    """res [records, B] recorded after ks[i] iterations -> per image the first recorded
+    iteration count with res <= target, or -1."""
     hit = res <= target[None, :]
-    first = np.where(hit.any(axis=0), hit.argmax(axis=0) + 1, -1)
-    return first
+    return np.where(hit.any(axis=0), ks[hit.argmax(axis=0)], -1)
+
+
+def _track(step, x, geo, b, x_true, start: int, max_iters: int, dense: int, stride: int, stop: float):
+    """Iterate step(x) (one SIRT step in place) from iteration `start` to `max_iters`, recording
+    the data residual and the FP64 image error of every image after iteration k for k <= dense
+    and every `stride` iterations beyond (and at the end).  Stops once every image's residual
+    is at or below `stop`.  Returns (ks, res [records, B], err [records, B])."""
+    B = x.shape[0]
+    xt_norm = x_true.reshape(B, -1).double().norm(dim=1)
+    ks, res, err = [], [], []
+
+    def record(k):
+        ks.append(k)
+        res.append(residual(geo, x, b).cpu().numpy())
+        err.append(((x - x_true).reshape(B, -1).double().norm(dim=1) / xt_norm).cpu().numpy())
+
+    record(start)
+    k = start
+    while k < max_iters and float(res[-1].max()) > stop:
+        step(x)
+        k += 1
+        if k <= dense or k % stride == 0 or k == max_iters:
+            record(k)
+    return np.asarray(ks), np.asarray(res), np.asarray(err)
 
 
 def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 300,
-             verbose: bool = True) -> dict:
+             verbose: bool = True, dense: int = 300, stride: int = 10,
+             tail_relax: float | None = None, seed: int = TEST_SEED) -> dict:
+    """Time to a data residual, learned vs plain SIRT, on `n_test` held-out phantoms.  Plain SIRT
+    (the reference's, relaxation 1) runs from zero; the learned method runs its stages, then SIRT
+    steps relaxed by its tail factor (the weights' `tail_relax`, or the argument).  Residuals and
+    image errors are recorded after every iteration up to `dense` and every `stride` iterations
+    beyond, up to `max_iters` or until every image is at or below 1e-2: an iteration count past
+    `dense` is an upper bound within `stride`."""
     geo = geometry(ctx)
     net = Correction(ctx)
     meta = net.load(weights)
     scale = float(meta["scale"])
     stages = int(meta["stages"]) if "stages" in meta else STAGES
-    model = LearnedSIRT(geo, net, stages)
-    rng = np.random.default_rng(TEST_SEED)
+    if tail_relax is None:
+        tail_relax = float(meta["tail_relax"]) if "tail_relax" in meta else 1.0
+    model = LearnedSIRT(geo, net, stages, tail_relax=tail_relax)
+    rng = np.random.default_rng(seed)
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED + 1)
-    # timing of one SIRT iteration and one stage (SIRT step + correction), batch of images
+    # timing of one SIRT iteration, one relaxed tail iteration and one stage (SIRT step +
+    # correction), batch of images
     xt = gpu_phantoms(ctx, rng, batch, N, ELLIPSES)
     b = noisy_sinogram(geo.fp(xt), scale, I0, gen)
     xb = torch.zeros_like(xt)
     with torch.no_grad():
         for _ in range(3):
             geo.sirt_(xb, b, 1)
-            net(model.sirt_step(xb, b))
+            model.tail_(xb, b, 1)
+            model.stage(xb, b, 1)
         model.reconstruct(b)  # warm-up
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         e[0].record()
         geo.sirt_(xb, b, 20)
         e[1].record()
         for _ in range(5):
-            net(model.sirt_step(xb, b))
+            model.stage(xb, b, 1)
         e[2].record()
         # the whole learned reconstruction as used: stages with the finiteness guard, the
         # plain SIRT steps of the fallback rule and both residuals
         model.reconstruct(b)
         e[3].record()
+        model.tail_(xb, b, 20)
+        e[4].record()
         torch.cuda.synchronize()
     t_iter = e[0].elapsed_time(e[1]) / 20 / batch * 1e-3
     t_stage = e[1].elapsed_time(e[2]) / 5 / batch * 1e-3
     t_recon = e[2].elapsed_time(e[3]) / batch * 1e-3
+    t_tail = e[3].elapsed_time(e[4]) / 20 / batch * 1e-3
 
     rows = []
     for start in range(0, n_test, batch):
@@ -193,59 +231,54 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         b = noisy_sinogram(geo.fp(x_true), scale, I0, gen)
         floor = residual(geo, x_true, b).cpu().numpy()
         with torch.no_grad():
-            # plain SIRT from zero
             x = torch.zeros_like(x_true)
-            res_p, err_p = [], []
-            for k in range(max_iters):
-                geo.sirt_(x, b, 1)
-                res_p.append(residual(geo, x, b).cpu().numpy())
-                err_p.append(((x - x_true).reshape(B, -1).double().norm(dim=1) /
-                              x_true.reshape(B, -1).double().norm(dim=1)).cpu().numpy())
-            # learned: stages, then plain SIRT continues from the learned image
+            kp, res_p, err_p = _track(lambda z: geo.sirt_(z, b, 1), x, geo, b, x_true, 0, max_iters,
+                                      dense, stride, 1e-2)
             xl, fell = model.reconstruct(b)
-            res_l = [residual(geo, xl, b).cpu().numpy()]
-            err_l = [((xl - x_true).reshape(B, -1).double().norm(dim=1) /
-                      x_true.reshape(B, -1).double().norm(dim=1)).cpu().numpy()]
-            for k in range(max_iters - stages):
-                geo.sirt_(xl, b, 1)
-                res_l.append(residual(geo, xl, b).cpu().numpy())
-                err_l.append(((xl - x_true).reshape(B, -1).double().norm(dim=1) /
-                              x_true.reshape(B, -1).double().norm(dim=1)).cpu().numpy())
-        res_p, err_p, res_l, err_l = map(np.asarray, (res_p, err_p, res_l, err_l))
+            kl, res_l, err_l = _track(lambda z: model.tail_(z, b, 1), xl, geo, b, x_true, 0,
+                                      max_iters - stages, dense, stride, 1e-2)
+        k5 = int(np.searchsorted(kp, stages))
         for tname, target in (("1e-2", np.full(B, 1e-2)), ("1.1x_floor", 1.1 * floor)):
-            kp = _first_hit(res_p, target)
-            kl = _first_hit(res_l, target)
+            hp = _first_hit(kp, res_p, target)
+            hl = _first_hit(kl, res_l, target)
             # image error plain SIRT has when it reaches the target, and the first learned
-            # state (after the stages, then after each extra plain iteration) at or below it
-            e_at = np.where(kp > 0, err_p[np.maximum(kp, 1) - 1, np.arange(B)], -1.0)
-            ke = _first_hit(err_l, e_at)
+            # state (after the stages, then after tail iterations) at or below it
+            ip = np.array([int(np.searchsorted(kp, h)) if h >= 0 else -1 for h in hp])
+            il = np.array([int(np.searchsorted(kl, h)) if h >= 0 else -1 for h in hl])
+            e_at = np.array([err_p[ip[i], i] if ip[i] >= 0 else -1.0 for i in range(B)])
+            he = _first_hit(kl, err_l, e_at)
             for i in range(B):
                 rows.append({"target": tname, "floor": float(floor[i]), "fell_back": bool(fell[i]),
-                             "plain_iters": int(kp[i]),
-                             "plain_err": float(err_p[kp[i] - 1, i]) if kp[i] > 0 else None,
-                             "learned_extra_iters": int(kl[i] - 1) if kl[i] > 0 else -1,
-                             "learned_err": float(err_l[kl[i] - 1, i]) if kl[i] > 0 else None,
-                             "plain_res_at_max": float(res_p[-1, i]),
+                             "plain_iters": int(hp[i]),
+                             "plain_err": float(err_p[ip[i], i]) if hp[i] >= 0 else None,
+                             "learned_extra_iters": int(hl[i]),
+                             "learned_err": float(err_l[il[i], i]) if hl[i] >= 0 else None,
+                             "plain_res_at_end": float(res_p[-1, i]), "plain_iters_run": int(kp[-1]),
                              "learned_res_after_stages": float(res_l[0, i]),
                              "learned_err_after_stages": float(err_l[0, i]),
-                             "plain_err_at_stages": float(err_p[stages - 1, i]),
-                             "learned_extra_iters_to_plain_err": int(ke[i] - 1) if kp[i] > 0 and ke[i] > 0 else -1})
+                             "plain_err_at_stages": float(err_p[min(k5, len(kp) - 1), i]),
+                             "learned_extra_iters_to_plain_err": int(he[i]) if hp[i] >= 0 else -1})
         if verbose:
             print(json.dumps({"done": start + B}), flush=True)
 
-    out = {"t_iter_ms": round(t_iter * 1e3, 4), "t_stage_ms": round(t_stage * 1e3, 4),
-           "t_reconstruct_ms": round(t_recon * 1e3, 4), "scale": scale,
-           "n_test": n_test, "max_iters": max_iters,
+    out = {"t_iter_ms": round(t_iter * 1e3, 4), "t_tail_iter_ms": round(t_tail * 1e3, 4),
+           "t_stage_ms": round(t_stage * 1e3, 4), "t_reconstruct_ms": round(t_recon * 1e3, 4),
+           "scale": scale, "tail_relax": tail_relax,
+           "stage_relax": [round(float(v), 4) for v in net.relax.detach().cpu()[1:stages]],
+           "n_test": n_test, "max_iters": max_iters, "dense_iters": dense, "stride": stride,
            "noise_floor_median": float(np.median([r["floor"] for r in rows if r["target"] == "1e-2"]))}
     for tname in ("1e-2", "1.1x_floor"):
         rs = [r for r in rows if r["target"] == tname]
-        both = [r for r in rs if r["plain_iters"] > 0 and r["learned_extra_iters"] >= 0]
-        summary = {"reached_plain": sum(r["plain_iters"] > 0 for r in rs),
+        both = [r for r in rs if r["plain_iters"] >= 0 and r["learned_extra_iters"] >= 0]
+        summary = {"reached_plain": sum(r["plain_iters"] >= 0 for r in rs),
                    "reached_learned": sum(r["learned_extra_iters"] >= 0 for r in rs), "of": len(rs)}
+        if tname == "1e-2":
+            summary["median_plain_res_at_end"] = round(float(np.median([r["plain_res_at_end"] for r in rs])), 5)
+            summary["plain_iters_run_max"] = max(r["plain_iters_run"] for r in rs)
         if both:
             tp = np.array([r["plain_iters"] * t_iter for r in both])
-            tl = np.array([t_recon + r["learned_extra_iters"] * t_iter for r in both])
-            ts = np.array([stages * t_stage + r["learned_extra_iters"] * t_iter for r in both])
+            tl = np.array([t_recon + r["learned_extra_iters"] * t_tail for r in both])
+            ts = np.array([stages * t_stage + r["learned_extra_iters"] * t_tail for r in both])
             summary.update({
                 "median_plain_ms": round(float(np.median(tp)) * 1e3, 3),
                 "median_learned_ms": round(float(np.median(tl)) * 1e3, 3),
@@ -255,12 +288,13 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                 "median_plain_iters": float(np.median([r["plain_iters"] for r in both])),
                 "median_learned_extra_iters": float(np.median([r["learned_extra_iters"] for r in both])),
                 "median_plain_err": round(float(np.median([r["plain_err"] for r in both])), 5),
-                "median_learned_err": round(float(np.median([r["learned_err"] for r in both])), 5)})
+                "median_learned_err": round(float(np.median([r["learned_err"] for r in both])), 5),
+                "learned_err_le_plain": sum(r["learned_err"] <= r["plain_err"] for r in both)})
             # the same image error as plain SIRT at its target, reached by the learned method
-            eq = [r for r in rs if r["plain_iters"] > 0 and r["learned_extra_iters_to_plain_err"] >= 0]
+            eq = [r for r in rs if r["plain_iters"] >= 0 and r["learned_extra_iters_to_plain_err"] >= 0]
             if eq:
                 tpe = np.array([r["plain_iters"] * t_iter for r in eq])
-                tle = np.array([t_recon + r["learned_extra_iters_to_plain_err"] * t_iter for r in eq])
+                tle = np.array([t_recon + r["learned_extra_iters_to_plain_err"] * t_tail for r in eq])
                 summary["equal_image_error"] = {
                     "of": len(eq), "median_learned_extra_iters": float(np.median(
                         [r["learned_extra_iters_to_plain_err"] for r in eq])),
@@ -274,6 +308,30 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                       "median_plain_err": round(float(np.median([r["plain_err_at_stages"] for r in rs])), 5),
                       "median_learned_res": round(float(np.median([r["learned_res_after_stages"] for r in rs])), 5)}
     return out
+
+
+def tune_tail(ctx, weights: str, n: int, candidates, batch: int = 16, max_iters: int = 300) -> dict:
+    """Pick the tail relaxation on validation phantoms (a seed apart from the training and test
+    streams): the factor with the shortest median learned time to 1.1x the noise floor whose
+    median image error there is not above plain SIRT's.  Stored in the weights' metadata."""
+    results = {}
+    for lam in candidates:
+        out = evaluate(ctx, weights, n, batch=batch, max_iters=max_iters, verbose=False,
+                       tail_relax=lam, seed=VALID_SEED)
+        t = out["1.1x_floor"]
+        results[lam] = t
+        print(json.dumps({"tail_relax": lam, "speedup_median": t.get("speedup_median"),
+                          "median_learned_err": t.get("median_learned_err"),
+                          "median_plain_err": t.get("median_plain_err"),
+                          "median_learned_extra_iters": t.get("median_learned_extra_iters")}), flush=True)
+    ok = [l for l, t in results.items() if t.get("reached_learned") == t.get("of") and
+          t.get("median_learned_err", 1e9) <= t.get("median_plain_err", -1)]
+    best = max(ok, key=lambda l: results[l]["speedup_median"]) if ok else 1.0
+    z = np.load(weights, allow_pickle=False)
+    arrs = {k: z[k] for k in z.files}
+    arrs["meta_tail_relax"] = np.asarray(best)
+    np.savez(weights, **arrs)
+    return {"tail_relax": best, "candidates": {str(k): v.get("speedup_median") for k, v in results.items()}}
 
 
 def main(argv=None) -> int:
@@ -296,16 +354,30 @@ def main(argv=None) -> int:
                    help="weight of the final image's projection error against the noise-free "
                         "data, ||A(x - x_true)||^2 / ||A x_true||^2")
     t.add_argument("--init", default="", help="start from these weights instead of a fresh init")
+    t.add_argument("--cin", type=int, default=1, choices=[1, 2],
+                   help="network input channels: 1 = the image after each SIRT step, 2 = also the "
+                        "step's update direction")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     e = sub.add_parser("eval")
     e.add_argument("--phantoms", type=int, default=1000)
     e.add_argument("--batch", type=int, default=16)
-    e.add_argument("--max-iters", type=int, default=300)
+    e.add_argument("--max-iters", type=int, default=3000)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    e.add_argument("--tail-relax", type=float, default=None,
+                   help="relaxation of the SIRT steps after the stages (default: the weights')")
+    tt = sub.add_parser("tune-tail")
+    tt.add_argument("--phantoms", type=int, default=64)
+    tt.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    tt.add_argument("--candidates", default="1.0,1.5,1.7,1.8,1.9")
     args = ap.parse_args(argv)
     if args.cmd == "train":
         return train(args)
-    res = evaluate(Context.default(0), args.weights, args.phantoms, args.batch, args.max_iters)
+    if args.cmd == "tune-tail":
+        res = tune_tail(Context.default(0), args.weights, args.phantoms,
+                        [float(v) for v in args.candidates.split(",")])
+    else:
+        res = evaluate(Context.default(0), args.weights, args.phantoms, args.batch, args.max_iters,
+                       tail_relax=args.tail_relax)
     print(json.dumps(res), flush=True)
     return 0
 

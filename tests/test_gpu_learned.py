@@ -106,6 +106,64 @@ def test_backward_matches_cpu_reference(ctx):
         assert err < 5e-4, (n, err)
 
 
+def test_two_channel_network_forward_and_backward(ctx):
+    """The correction with two input channels (the image and its SIRT update direction,
+    sk_conv_in2): forward against the CPU FP32 reference with the kernels' BF16 rounding points,
+    and the gradients w.r.t. both inputs and the first layer against FP64 autograd (with the
+    kernels' own ReLU masks)."""
+    from synkrotomo_amd import _native as nat
+    from synkrotomo_amd.learned import Correction
+    from synkrotomo_amd.projector import _ptr, _stream
+    net = Correction(ctx, seed=5, cin=2)
+    with torch.no_grad():
+        net.w4.copy_(torch.randn(net.w4.shape, generator=torch.Generator().manual_seed(9)).cuda() * 0.05)
+        for b in (net.b1, net.b2, net.b3, net.b4):
+            b.copy_(torch.randn(b.shape, generator=torch.Generator().manual_seed(4)).cuda() * 0.1)
+    gen = torch.Generator(device="cuda").manual_seed(12)
+    x = torch.rand(2, 40, 36, device="cuda", generator=gen).requires_grad_(True)
+    d = (torch.rand(2, 40, 36, device="cuda", generator=gen) - 0.5).requires_grad_(True)
+    gy = torch.randn(2, 40, 36, device="cuda", generator=gen)
+    y = net(x, d)
+    cpu = lambda t: t.detach().cpu()
+    w1, w2, w3, w4 = (_bf(cpu(w)) for w in (net.w1, net.w2, net.w3, net.w4))
+    b1, b2, b3, b4 = (cpu(b) for b in (net.b1, net.b2, net.b3, net.b4))
+    inp = torch.stack([cpu(x), cpu(d)], dim=1)                          # [B, 2, H, W]
+    c1 = lambda i, w, b: F.conv2d(i, w.permute(0, 3, 1, 2).contiguous(), b, padding=1)
+    c32 = lambda i, w, b: F.conv2d(i, w.permute(0, 3, 1, 2).contiguous(), b, padding=1)
+    c4 = lambda i, w, b: F.conv2d(i, w.permute(2, 0, 1).unsqueeze(0).contiguous(), b, padding=1)
+    r1 = _bf(F.relu(c1(inp, w1, b1)))
+    r2 = _bf(F.relu(c32(r1, w2, b2)))
+    r3 = _bf(F.relu(c32(r2, w3, b3)))
+    ref_y = (cpu(x).unsqueeze(1) + c4(r3, w4, b4)).squeeze(1)
+    assert rel_l2(cpu(y).numpy(), ref_y.numpy()) < 1e-3
+    grads = torch.autograd.grad(y, [x, d, net.w1, net.b1], gy)
+    # the kernels' own activations (BF16 NHWC) for the masks and the layer inputs
+    L, h, s = nat.lib(), ctx.handle, _stream(ctx.device)
+    B, H, W = x.shape
+    x2 = torch.stack([x.detach(), d.detach()], dim=-1).contiguous()
+    w1b, w2b, w3b = (w.detach().to(torch.bfloat16).contiguous() for w in (net.w1, net.w2, net.w3))
+    h1 = torch.empty((B, H, W, 32), dtype=torch.bfloat16, device="cuda")
+    h2, h3 = torch.empty_like(h1), torch.empty_like(h1)
+    nat.check(L.sk_conv_in2(h, B, H, W, _ptr(x2), _ptr(w1b), _ptr(net.b1.detach()), _ptr(h1), s), h, "conv1")
+    nat.check(L.sk_conv32(h, B, H, W, _ptr(h1), None, _ptr(w2b), _ptr(net.b2.detach()), _ptr(h2), s), h, "conv2")
+    nat.check(L.sk_conv32(h, B, H, W, _ptr(h2), None, _ptr(w3b), _ptr(net.b3.detach()), _ptr(h3), s), h, "conv3")
+    nchw = lambda t: t.float().permute(0, 3, 1, 2).contiguous().cpu()
+    h1, h2, h3 = nchw(h1), nchw(h2), nchw(h3)
+    g = cpu(gy).unsqueeze(1)
+    dh3, _, _ = _lin_grads(c4, h3, w4, b4, g)
+    dp3 = _bf(dh3.float()) * (h3 > 0)
+    dh2, _, _ = _lin_grads(c32, h2, w3, b3, dp3)
+    dp2 = _bf(dh2.float()) * (h2 > 0)
+    dh1, _, _ = _lin_grads(c32, h1, w2, b2, dp2)
+    dp1 = _bf(dh1.float()) * (h1 > 0)
+    dinp, gw1, gb1 = _lin_grads(c1, inp, w1, b1, dp1)
+    ref = [dinp[:, 0] + g.double().squeeze(1), dinp[:, 1], gw1, gb1]
+    for n, gg, r in zip(["x", "d", "w1", "b1"], grads, ref):
+        err = rel_l2(gg.cpu().numpy(), r.numpy())
+        print(f"two-channel grad {n}: rel-L2 {err:.2e}")
+        assert err < 5e-4, (n, err)
+
+
 def test_conv32_is_mfma_gemm_exactly_on_bf16_inputs(ctx):
     """With BF16-exact operands and sums the MFMA conv equals an FP64 convolution exactly."""
     from synkrotomo_amd import _native as nat
@@ -119,6 +177,27 @@ def test_conv32_is_mfma_gemm_exactly_on_bf16_inputs(ctx):
     ref = F.conv2d(x.double().cpu().permute(0, 3, 1, 2), w.double().cpu().permute(0, 3, 1, 2), padding=1)
     np.testing.assert_array_equal(y.float().cpu().permute(0, 3, 1, 2).numpy(),
                                   ref.to(torch.bfloat16).float().numpy())
+
+
+def test_wgrad32_mfma_is_exact_on_small_integers(ctx):
+    """The 32 -> 32 weight and bias gradients (MFMA, K = pixels) on BF16-exact small-integer
+    operands, whose sums are exact in FP32 in any order: equal to the FP64 result exactly, with
+    the ReLU mask applied, edge tiles on both axes (40 x 36) and two images."""
+    from synkrotomo_amd import _native as nat
+    from synkrotomo_amd.projector import _ptr, _stream
+    B, H, W = 2, 40, 36
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    g = torch.randint(-2, 3, (B, H, W, 32), device="cuda", generator=gen).to(torch.bfloat16)
+    mk = torch.randint(-1, 2, (B, H, W, 32), device="cuda", generator=gen).to(torch.bfloat16)
+    x = torch.randint(-2, 3, (B, H, W, 32), device="cuda", generator=gen).to(torch.bfloat16)
+    out = torch.empty(32 * 9 * 32 + 32, device="cuda")
+    nat.check(nat.lib().sk_conv_wgrad(ctx.handle, 1, B, H, W, _ptr(g), _ptr(mk), _ptr(x), _ptr(out),
+                                      _stream(0)), ctx.handle)
+    gm = (g.double() * (mk.double() > 0)).cpu().permute(0, 3, 1, 2)
+    xc = x.double().cpu().permute(0, 3, 1, 2)
+    dw = torch.nn.grad.conv2d_weight(xc, (32, 32, 3, 3), gm, padding=1)  # [co][ci][ky][kx]
+    ref = torch.cat([dw.permute(0, 2, 3, 1).reshape(-1), gm.sum(dim=(0, 2, 3))])
+    np.testing.assert_array_equal(out.cpu().double().numpy(), ref.numpy())
 
 
 def test_gpu_phantoms_equal_numpy_generator(ctx):
