@@ -87,6 +87,15 @@ int sk_lines_compute(const float* angles_host, int A, float* table_out);
  * d_img [batch][N][N] -> d_sino [batch][A][R]. */
 int sk_fp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_img, float* d_sino, void* stream);
 
+/* A.x with no device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels and results as sk_fp; the transposed image that the flat
+ * lines read goes to a fixed 64 MiB device array of the library instead of the context's
+ * growable scratch, so batch * N * N must be at most 16,777,216 floats when any angle is flat
+ * (SK_EINVAL otherwise).  One array per GPU, shared by every call on it: calls that overlap on
+ * one GPU race. */
+int sk_fp_noalloc(sk_ctx* ctx, sk_geom* g, int batch, const float* d_img, float* d_sino, void* stream);
+
 /* B.r: backprojection.main (futhark/backprojection.fut:82-93).
  * d_sino [batch][A][R] -> d_img [batch][N][N]. */
 int sk_bp(sk_ctx* ctx, sk_geom* g, int batch, const float* d_sino, float* d_img, void* stream);

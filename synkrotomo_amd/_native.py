@@ -39,6 +39,7 @@ SIGNATURES = {
     "sk_lines_compute": (_i, [_f32p, _i, _f32p]),
     "sk_compare": (_i, [_vp, ctypes.c_int64, _vp, _vp, _f, _ip, _i64p, _f32p, _vp]),
     "sk_fp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "sk_fp_noalloc": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp_adjoint": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "sk_bp_transpose": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),

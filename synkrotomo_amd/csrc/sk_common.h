@@ -108,6 +108,10 @@ int hip_fail(Ctx* c, hipError_t e, const char* what);
 // Scratch memory of at least `bytes`, ordered after `stream`.
 int scratch(Ctx* c, size_t bytes, hipStream_t stream, void** out);
 
+// Device address of sk_fp_noalloc's fixed scratch on `device` (current), looked up once per
+// device (sk_noalloc.hip).
+int noalloc_init(Ctx* c, int device);
+
 // Kernel launchers (sk_proj.hip).  batch >= 1; strides are per-batch element counts.
 enum FpMode { FP_PLAIN = 0, FP_ONES_INV = 1, FP_SIRT = 2 };
 enum BpMode { BP_PLAIN = 0, BP_ONES_INV = 1, BP_SIRT = 2 };

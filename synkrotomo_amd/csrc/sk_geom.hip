@@ -132,6 +132,11 @@ int sk_create(int device, sk_ctx** out) {
     delete c;
     return SK_EHIP;
   }
+  if (noalloc_init(&c->c, device) != SK_OK) {
+    (void)hipStreamDestroy(c->c.capture_stream);
+    delete c;
+    return SK_EHIP;
+  }
   *out = c;
   return SK_OK;
 }
