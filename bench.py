@@ -9,9 +9,10 @@ centred on the image (rhozero = -1448): the configuration the headline metric is
 on.  Inputs are resident in HBM before timing.  Prints ONE JSON line.
 
 value      = GUPS = 2 * N^2 * A * K / (wall time of K steps) / 1e9   (fwd and back updates)
-roofline   = the dominant kernel's algorithmic bytes per launch (4*nnz + 4*(N^2 + A*R),
-             nnz = 10,937,047,089 counted exactly by the oracle) / its mean launch time
-             (HIP events on the launch stream), against 8 TB/s HBM
+roofline   = VALU issue of the dominant kernel: its VALU wave-instructions per launch
+             (SQ_INSTS_VALU, profiles/counters.json) / its mean launch time (HIP events on the
+             launch stream), against 2 per clock per CU (the published FP32 vector rate); the
+             HBM bytes it moves (FETCH_SIZE + WRITE_SIZE) and their fraction of 8 TB/s beside it
 cpu_baseline = the oracle (C restatement of the reference, OpenMP) on a bounded sample
              of the same workload on this host's cores
 """
@@ -23,12 +24,18 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 N, A, R, RHO0 = 2048, 2048, 2897, -1448.0
 NNZ_C2 = 10_937_047_089          # exact nnz(A) at this geometry (oracle CSR count)
 HBM_PEAK_GBS = 8000.0            # MI355X published HBM3E bandwidth
+# VALU issue peak: the published 157.3 TFLOP/s FP32 vector rate is 256 CUs x 2.4 GHz x 2 wave64
+# VALU instructions per clock per CU (gfx950 issues two VALU instructions per SIMD quad-cycle,
+# counter SQ_ACTIVE_INST_VALU2) x 64 lanes x 2 FLOP (FMA); in wave64 VALU instructions per second:
+VALU_PEAK_GINST = 157.3e12 / (64 * 2) / 1e9   # 1228.9
 PHANTOM_SEED, PHANTOM_ELLIPSES = 2048, 50
 
 
@@ -36,25 +43,100 @@ def alg_bytes() -> float:
     return 4.0 * NNZ_C2 + 4.0 * (N * N + A * R)
 
 
-def cpu_baseline(angles, phantom, sino) -> dict:
-    """Oracle fp + bp on every 4th angle (512 of 2048) at 2048^2, all host cores."""
-    import numpy as np
+def roofline_of(dom: str, ms: dict, exact: bool, variants) -> dict:
+    """VALU-issue roofline of the dominant kernel (and the other one beside it): its VALU wave
+    instructions per launch (SQ_INSTS_VALU) and HBM bytes per launch (FETCH_SIZE and WRITE_SIZE,
+    calibrated on a copy kernel), both read from profiles/counters.json (the rocprofv3 passes
+    over this benchmark's own steps, kernels and variants named there), divided by the launch
+    time measured live here with HIP events on the launch stream.  The counts are properties of
+    the kernel and the geometry (no data-dependent control flow), so they hold across runs of
+    the same kernel variant; a run on other variants or in exact mode reports no counters."""
+    path = os.path.join(ROOT, "profiles", "counters.json")
+    cnt = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            cnt = json.load(f)
+    out = {}
+    for name in ("fp", "bp"):
+        k = cnt.get(name, {})
+        ok = (not exact and variants is not None and k.get("variant") == variants[0 if name == "fp" else 1])
+        t = ms[name] * 1e-3
+        e = {"kernel": k.get("kernel") if ok else None, "ms": round(ms[name], 4)}
+        if ok:
+            valu = float(k["valu_insts_per_launch"])
+            hbm = float(k["hbm_bytes_per_launch"])
+            e.update({"valu_insts_per_launch": valu,
+                      "valu_ginst_s": round(valu / t / 1e9, 1),
+                      "valu_frac": round(valu / t / 1e9 / VALU_PEAK_GINST, 4),
+                      "hbm_bytes_per_launch": hbm,
+                      "hbm_GBs": round(hbm / t / 1e9, 1),
+                      "hbm_frac": round(hbm / t / 1e9 / HBM_PEAK_GBS, 4),
+                      "alg_bytes_GBs": round(alg_bytes() / t / 1e9, 1)})
+        out[name] = e
+    d = out[dom]
+    return {"bound": "valu", "kernel": dom,
+            "achieved": d.get("valu_ginst_s"), "peak": round(VALU_PEAK_GINST, 1), "unit": "Gwave-inst/s",
+            "frac": d.get("valu_frac"), "traffic": d.get("hbm_bytes_per_launch"),
+            "hbm_frac": d.get("hbm_frac"), "alg_bytes_per_launch": alg_bytes(),
+            "peak_source": "157.3 TFLOP/s FP32 vector / (64 lanes x 2 FLOP per FMA): 2 VALU "
+                           "wave-instructions per clock per CU at 2.4 GHz on 256 CUs",
+            "counters": "profiles/counters.json", "fp": out["fp"], "bp": out["bp"]}
+
+
+def host_threads() -> tuple[int, int]:
+    """(threads used, cores in this process's affinity mask).  The oracle runs on every core of
+    the mask unless OMP_NUM_THREADS names the host's CPU share (the GPU hosts here give each GPU
+    a share of 16 cores and set it)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else aff), aff
+
+
+def _median_time(fn, runs: int):
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(angles, phantom, sino, runs: int = 3) -> dict:
+    """The oracle (C restatement of the reference, OpenMP over rays / pixels) on the host:
+    fp over all 2048 angles; bp over every 4th angle (512 of 2048; its cost is linear in the
+    angle count) scaled to 2048; SIRT per iteration from the difference of a 3- and a 1-iteration
+    run on every 8th angle (256 angles) scaled to 2048.  Median of `runs` for fp and bp."""
     import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    nt, aff = host_threads()
+    say = lambda m: print(f"[bench] cpu baseline: {m}", file=sys.stderr, flush=True)
+    say(f"{nt} threads (affinity {aff}), fp 2048 angles x{runs}")
+    fp_s, fp_all = _median_time(lambda: oracle.fp(angles, RHO0, 1.0, R, phantom, nthreads=nt), runs)
     sub = np.arange(0, A, 4)
-    ang = angles[sub]
-    sino_sub = sino.reshape(A, R)[sub].ravel()
+    ang4, sino4 = angles[sub], sino.reshape(A, R)[sub].ravel()
+    say(f"bp 512 angles x{runs}")
+    bp4_s, bp_all = _median_time(lambda: oracle.bp(ang4, RHO0, 1.0, N, sino4, nthreads=nt), runs)
+    bp_s = bp4_s * A / ang4.size
+    sub8 = np.arange(0, A, 8)
+    ang8, sino8 = angles[sub8], sino.reshape(A, R)[sub8].ravel()
+    zero = np.zeros(N * N, np.float32)
+    say("SIRT 1 and 3 iterations on 256 angles")
     t0 = time.perf_counter()
-    oracle.fp(ang, RHO0, 1.0, R, phantom, nthreads=cores)
+    oracle.sirt(ang8, RHO0, 1.0, zero, sino8, 1, nthreads=nt)
     t1 = time.perf_counter()
-    oracle.bp(ang, RHO0, 1.0, N, sino_sub, nthreads=cores)
+    oracle.sirt(ang8, RHO0, 1.0, zero, sino8, 3, nthreads=nt)
     t2 = time.perf_counter()
-    updates = 2.0 * N * N * ang.size
-    return {"value": round(updates / (t2 - t0) / 1e9, 3), "unit": "GUPS", "cores": cores, "kind": "port",
-            "sample": f"oracle fp+bp at 2048^2, {ang.size} of {A} angles (every 4th), 2897 bins; "
-                      f"fp {t1 - t0:.2f} s, bp {t2 - t1:.2f} s",
-            "fp_gups": round(N * N * ang.size / (t1 - t0) / 1e9, 3),
-            "bp_gups": round(N * N * ang.size / (t2 - t1) / 1e9, 3)}
+    it8 = ((t2 - t1) - (t1 - t0)) / 2.0
+    upd = float(N) * N * A
+    return {"value": round(2.0 * upd / (fp_s + bp_s) / 1e9, 3), "unit": "GUPS", "cores": nt,
+            "affinity_cores": aff, "kind": "port", "runs": runs,
+            "sample": (f"oracle at 2048^2, 2897 bins, {nt} threads: fp over all 2048 angles (median of "
+                       f"{runs}); bp over every 4th angle, scaled x4 (median of {runs}); SIRT per "
+                       "iteration from 3- minus 1-iteration runs over every 8th angle, scaled x8"),
+            "fp_s": round(fp_s, 3), "bp_s": round(bp_s, 3),
+            "fp_runs_s": [round(t, 3) for t in fp_all], "bp_512_runs_s": [round(t, 3) for t in bp_all],
+            "fp_gups": round(upd / fp_s / 1e9, 3), "bp_gups": round(upd / bp_s / 1e9, 3),
+            "sirt_ms_per_iter": round(it8 * 1e3 * A / ang8.size, 1),
+            "sirt_ms_per_iter_256_angles": round(it8 * 1e3, 1)}
 
 
 def sparse_leg(geo, x, sino, K: int) -> dict:
@@ -304,14 +386,16 @@ def main() -> int:
     ap.add_argument("--dense-only", action="store_true",
                     help="only the dense fp/bp steps (every side leg and the CPU baseline skipped)")
     ap.add_argument("--no-prep", action="store_true", help="skip the normalisation leg")
+    ap.add_argument("--only", choices=["sirt", "stream", "prep", "learned", "batched", "sparse", "cpu"],
+                    help="the dense steps plus this one side leg (profiling runs)")
     ap.add_argument("--batched-iters", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use default kernel variants")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device copy first (reference dispatch for HBM counters)")
     args = ap.parse_args()
-    if args.dense_only:
+    if args.dense_only or args.only:
         for leg in ("cpu", "sparse", "batched", "learned", "sirt", "stream", "prep"):
-            setattr(args, f"no_{leg}", True)
+            setattr(args, f"no_{leg}", leg != args.only)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the projector shards by "
               "angle across GPUs, which this build does not run.", file=sys.stderr)
@@ -364,12 +448,7 @@ def main() -> int:
     upd = float(N) * N * A
     value = 2.0 * upd * K / wall / 1e9
     dom_name, dom_ms = ("bp", bp_ms) if bp_ms >= fp_ms else ("fp", fp_ms)
-    achieved = alg_bytes() / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom_name)
+    roofline = roofline_of(dom_name, {"fp": fp_ms, "bp": bp_ms}, args.exact, variants)
 
     out = {
         "metric": "GUPS fwd/back @2048²; time-to-residual 1e-2 learned vs plain SIRT",
@@ -390,9 +469,7 @@ def main() -> int:
                    "variants": {"fp": variants[0], "bp": variants[1]} if variants else "default"},
         "fp_ms": round(fp_ms, 3), "bp_ms": round(bp_ms, 3),
         "fp_gups": round(upd / (fp_ms * 1e-3) / 1e9, 2), "bp_gups": round(upd / (bp_ms * 1e-3) / 1e9, 2),
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes()},
+        "roofline": roofline,
         "cpu_baseline": None,
         "sirt": None,
         "stream3d": None,
