@@ -242,8 +242,14 @@ int sk_mat_copy(sk_ctx* ctx, const sk_mat* m, int64_t* d_rowptr, int32_t* d_cols
  * of the row (6 bytes per entry instead of 8; for SK_MAT_BP / SK_MAT_ADJ the steps run over the
  * rays in line-table order, steep lines then flat lines, and each call first permutes the
  * input's rows into that order; unavailable, and sk_spmv fails with EINVAL, when two
- * consecutive columns of a row are more than 32767 apart).  Kernels that sum a row in the
- * same order give the same bits: 0, 3; 2, 4; 1, 7, 8, 9.  -1 (default) = time the candidates
+ * consecutive columns of a row are more than 32767 apart); 10 = row slots, one lane per row on a
+ * copy that keeps the 64 rows of a slice in step: for SK_MAT_FP a slice is 64 bins of one angle
+ * and record k holds each ray's two entries in frame row k (gathered from the image, or its
+ * transpose for flat angles: one frame row per wave), for SK_MAT_BP a slice is 64 pixels of an
+ * image row and record j holds each pixel's two candidates of the line at table position j
+ * (deltarho in [0.71, 1.41) only); holes where an entry does not exist; built on first use (not
+ * for SK_MAT_ADJ: EINVAL).  Kernels that sum a row in the same order give the same bits: 0, 3;
+ * 2, 4; 1, 7, 8, 9, 10.  -1 (default) = time the candidates
  * (0, 2; 3, 4 for SK_MAT_FP; 7, 8 for SK_MAT_FP and 9, if their copies fit in half the free
  * device memory) on the first sk_spmv call and keep the fastest.  EINVAL for 5, 6, and for 3,
  * 4, 8 on other matrices. */

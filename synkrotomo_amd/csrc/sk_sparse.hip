@@ -37,7 +37,8 @@ struct sk_mat {
   int spmv_variant = -1;  // 0: wave per row, 1: lane per row (lockstep), 2: wave per row with
                           // 16-byte streams; 3, 4: as 0, 2 with the gathers of steep rays
                           // from x^T (SK_MAT_FP only); 7: SELL-64 copy; 9: SELL-64 with
-                          // 16-bit column steps; -1: not yet timed
+                          // 16-bit column steps; 10: row slots (SK_MAT_FP only); -1: not
+                          // yet timed
   int64_t rows = 0, cols = 0, nnz = 0;
   // SK_MAT_FP: per original angle, 1 if its line is steep (its rays step through image rows:
   // in x^T their pixels are contiguous), and the geometry's R and N
@@ -67,6 +68,28 @@ struct sk_mat {
   int32_t* line_pos = nullptr;    // SK_MAT_BP / SK_MAT_ADJ: [A] table position of each angle,
   int32_t* line_orig = nullptr;   //   then [A] the angle at each table position
   int A = 0;
+  // Variant 10 (SK_MAT_FP): row slots.  Slice s = caller's angle a x block kb of 64 bins (rays
+  // a*R + 64kb + l, one per lane l); its records k = 0 .. rs_len-1 are the frame rows
+  // rs_lo + k.  Record (s, k, l) holds the ray's two entries of that row, (lb, lt) at
+  // rs_val[(rs_off[s] + k)*64 + l], NaN where an entry does not exist, and a control byte in byte
+  // k%4 of rs_ctrl[(rs_off[s]/4 + k/4)*64 + l]: low nibble Xb(i) - Xb(i-1), high nibble
+  // Xt(i) - Xb(i), both signed 4-bit.  The frame index (i + N/2)*N + Xb + N/2 of record k is
+  // rs_first[s*64 + l] + (k+1)*N + the nibble sum: the lanes of a wave gather from one frame row
+  // (the image for steep angles, its transpose for flat ones) instead of drifting apart.
+  int64_t* rs_off = nullptr;    // [slices + 1], a multiple of 8 records per slice
+  uint32_t* rs_ctrl = nullptr;
+  float2* rs_val = nullptr;
+  int32_t* rs_first = nullptr;  // [slices * 64]
+  int32_t* rs_lpos = nullptr;   // [A] table position of each angle's line
+  sk::Line* rs_lines = nullptr; // the geometry's line table (the slots are built on first use)
+  float rho0 = 0.0f, drho = 1.0f;
+  int ncand = 0;
+  // For SK_MAT_BP the row slots are per line instead: slice s = image row Y x block xb of 64
+  // pixels (pixels Y*N + 64xb + l), record j = the line at table position j (A rounded up to 8
+  // records), holding the pixel's two candidates (s', s'+1) of that line, rs_val as above; the
+  // control is a 16-bit step of the base column j*R + s' (line-table order) per record, two per
+  // word: word (s*A8/2 + j/2)*64 + l; rs_first holds the base column before record 0.
+  bool rs_failed = false;
   int64_t* rowptr = nullptr;  // [rows + 1]
   int32_t* col = nullptr;     // [nnz]
   float* val = nullptr;       // [nnz]
@@ -733,6 +756,324 @@ __global__ __launch_bounds__(256) void k_spmv_lockstep(const int64_t* __restrict
   if (row < rows) y[(size_t)blockIdx.y * rows + row] = acc;
 }
 
+// ------------------------------------------------------------------ row slots (variant 10, A)
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// The row step of one ray (k_fp_rows's expressions: forwardprojection.fut:31-54).
+struct RsStep {
+  int Xb, Xt;
+  bool inb, intop;
+  float lb, lt;
+};
+__device__ __forceinline__ RsStep rs_step(int i, int h, float slope, float icpt, float lbase, bool live) {
+  RsStep t;
+  const float xbot = (float)i * slope + icpt;
+  const float xtop = xbot + slope;
+  t.Xb = (int)floorf(xbot);
+  t.Xt = (int)floorf(xtop);
+  const bool single = t.Xb == t.Xt;
+  t.inb = live && t.Xb >= -h && t.Xb < h;
+  t.intop = live && !single && t.Xt >= -h && t.Xt < h;
+  const float Xmax = (float)(t.Xb > t.Xt ? t.Xb : t.Xt);
+  const float xdiff = xtop - xbot;
+  t.lb = single ? lbase : ((Xmax - xbot) / xdiff) * lbase;
+  t.lt = ((xtop - Xmax) / xdiff) * lbase;
+  return t;
+}
+
+// One wave per slice: the frame rows [lo, lo + len) holding an entry of any of its rays (len a
+// multiple of 8; 0 for a slice without entries).
+__global__ __launch_bounds__(SP_BLOCK) void k_rs_count(const Line* __restrict__ lines, const int32_t* __restrict__ lpos,
+                                                       int A, int R, int N, float rho0, float drho, int nbb,
+                                                       int32_t* __restrict__ lo_out, int32_t* __restrict__ len_out) {
+  const long s = (long)blockIdx.x * (SP_BLOCK / 64) + (threadIdx.x >> 6);
+  if (s >= (long)A * nbb) return;  // whole wave
+  const int lane = lane_id();
+  const int a = (int)(s / nbb), r = (int)(s - (long)a * nbb) * 64 + lane;
+  const Line L = lines[lpos[a]];
+  const int h = N >> 1;
+  int first = 1 << 30, last = -(1 << 30);
+  if (r < R) {
+    const float icpt = (rho0 + (float)r * drho) / L.cost;  // forwardprojection.fut:33-34
+    for (int i = -h; i < h; ++i) {
+      const RsStep t = rs_step(i, h, L.slope, icpt, L.lbase, true);
+      if (t.inb || t.intop) {
+        first = min(first, i);
+        last = i;
+      }
+    }
+  }
+  first = wave_min(first);
+  last = wave_max(last);
+  if (lane == 0) {
+    lo_out[s] = first <= last ? first : 0;
+    len_out[s] = first <= last ? ((last - first + 1 + 7) & ~7) : 0;
+  }
+}
+
+// One wave per slice: fill its records (lane = ray).  Sets *bad when a nibble does not fit.
+__global__ __launch_bounds__(SP_BLOCK) void k_rs_fill(const Line* __restrict__ lines, const int32_t* __restrict__ lpos,
+                                                      int A, int R, int N, float rho0, float drho, int nbb,
+                                                      const int32_t* __restrict__ lo, const int64_t* __restrict__ off,
+                                                      uint32_t* __restrict__ ctrl, float2* __restrict__ val,
+                                                      int32_t* __restrict__ first, unsigned* __restrict__ bad) {
+  const long s = (long)blockIdx.x * (SP_BLOCK / 64) + (threadIdx.x >> 6);
+  if (s >= (long)A * nbb) return;
+  const int lane = lane_id();
+  const int a = (int)(s / nbb), r = (int)(s - (long)a * nbb) * 64 + lane;
+  const Line L = lines[lpos[a]];
+  const int h = N >> 1;
+  const long o = off[s];
+  const int len = (int)(off[s + 1] - o);
+  const int i0 = lo[s];
+  const bool valid = r < R;
+  const float icpt = valid ? (rho0 + (float)r * drho) / L.cost : 0.0f;
+  const float nan = __builtin_nanf("");
+  bool over = false;
+  int prev = 0;
+  uint32_t word = 0;
+  for (int k = 0; k < len; ++k) {
+    const int i = i0 + k;
+    int dX = 0, ot = 0;
+    float vb = nan, vt = nan;
+    if (valid) {
+      const RsStep t = rs_step(i, h, L.slope, icpt, L.lbase, i < h);
+      if (k == 0) first[s * 64 + lane] = (i + h) * N + (t.Xb + h) - N;
+      dX = k == 0 ? 0 : t.Xb - prev;
+      ot = t.Xt - t.Xb;
+      prev = t.Xb;
+      if (t.inb) vb = t.lb;
+      if (t.intop) vt = t.lt;
+      over = over || dX < -8 || dX > 7 || ot < -8 || ot > 7;
+    } else if (k == 0) {
+      first[s * 64 + lane] = -N;
+    }
+    word |= ((uint32_t)(dX & 15) | ((uint32_t)(ot & 15) << 4)) << (8 * (k & 3));
+    if ((k & 3) == 3) {
+      ctrl[(o / 4 + k / 4) * 64 + lane] = word;
+      word = 0;
+    }
+    val[(o + k) * 64 + lane] = make_float2(vb, vt);
+  }
+  if (len == 0) first[s * 64 + lane] = -N;
+  if (over) atomicOr(bad, 1u);
+}
+
+__device__ __forceinline__ int nib(uint32_t w, int sh) { return ((int)(w << (28 - sh))) >> 28; }
+typedef float vf2 __attribute__((ext_vector_type(2)));
+
+// y = A x on the row slots: one wave per slice, one ray per lane, records in row order, the
+// entries of a record in the order Xb, Xt -- the CSR order, so the sums are bitwise those of
+// variants 1, 7 and 9.  Gathers through a buffer descriptor over the frame image: frame
+// indices outside it (a hole's neighbour) read 0.  Eight records per step: the next step's
+// control words are loaded first and its values after them, while this step's sixteen gathers
+// are summed.
+__global__ __launch_bounds__(256) void k_spmv_rs(const int64_t* __restrict__ off, const uint32_t* __restrict__ ctrl,
+                                                 const float2* __restrict__ val, const int32_t* __restrict__ first,
+                                                 const unsigned char* __restrict__ steep, long slices, int nbb, int R,
+                                                 int N, long ncols, long nrows, const float* __restrict__ x,
+                                                 const float* __restrict__ xT, float* __restrict__ y) {
+  // the slice, its angle and frame are wave-uniform (scalar registers: the buffer descriptor
+  // below must be uniform)
+  const long s = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (s >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const int a = (int)(s / nbb), r = (int)(s - (long)a * nbb) * 64 + lane;
+  const int st = __builtin_amdgcn_readfirstlane((int)steep[a]);
+  const float* src = (st ? x : xT) + (size_t)blockIdx.y * ncols;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)src, 0, (int)(ncols * 4), 0x00020000u);
+  const long o = off[s];
+  const int len = (int)(off[s + 1] - o);  // a multiple of 8
+  const uint32_t* cp = ctrl + (o / 4) * 64 + lane;
+  const vf2* vp = (const vf2*)(val + o * 64 + lane);
+  int cur = first[s * 64 + lane];
+  float acc = 0.0f;
+  if (len > 0) {
+    uint32_t w[2];
+    vf2 v[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) w[h] = __builtin_nontemporal_load(cp + 64 * h);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(vp + 64 * u);
+    for (int k = 0; k < len; k += 8) {
+      float gb[8], gt[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t ww = w[u >> 2];
+        const int sh = 8 * (u & 3);
+        cur += N + nib(ww, sh);
+        const int ct = cur + nib(ww, sh + 4);
+        gb[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, cur << 2, 0, 0));
+        gt[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ct << 2, 0, 0));
+      }
+      const bool more = k + 8 < len;  // uniform over the wave
+      uint32_t wn[2];
+      vf2 vn[8];
+      __builtin_amdgcn_sched_barrier(0);
+      if (more) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) wn[h] = __builtin_nontemporal_load(cp + 64 * ((k + 8) / 4 + h));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vn[u] = __builtin_nontemporal_load(vp + 64 * (k + 8 + u));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (v[u].x == v[u].x) acc = fmaf(v[u].x, gb[u], acc);
+        if (v[u].y == v[u].y) acc = fmaf(v[u].y, gt[u], acc);
+      }
+      if (more) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) w[h] = wn[h];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = vn[u];
+      }
+    }
+  }
+  if (r < R) y[(size_t)blockIdx.y * nrows + (size_t)a * R + r] = acc;
+}
+
+// ------------------------------------------------------------------ row slots for B (variant 10)
+// One wave per slice (64 pixels of an image row), lanes = pixels, records = lines in table order:
+// the two candidates of backprojection.fut:44-65 with k_bp_rows's expressions (so the values and
+// the member test are the CSR's).  Sets *bad when a column step does not fit in 16 bits.
+__global__ __launch_bounds__(SP_BLOCK) void k_rsb_fill(const Line* __restrict__ lines, int A, int A8, int R, int N,
+                                                       float rho0, float drho, uint32_t* __restrict__ ctrl,
+                                                       float2* __restrict__ val, int32_t* __restrict__ first,
+                                                       unsigned* __restrict__ bad) {
+  const int nxb = (N + 63) / 64;
+  const long sl = (long)blockIdx.x * (SP_BLOCK / 64) + (threadIdx.x >> 6);
+  if (sl >= (long)N * nxb) return;
+  const int lane = lane_id();
+  const int Y = (int)(sl / nxb), X = (int)(sl - (long)Y * nxb) * 64 + lane;
+  const bool valid = X < N;
+  const int h = N >> 1;
+  const long o = sl * A8;
+  const float nan = __builtin_nanf("");
+  bool over = false;
+  long prev = 0;
+  uint32_t word = 0;
+  for (int j = 0; j < A8; ++j) {
+    long step = 0;
+    float v0 = nan, v1 = nan;
+    if (valid && j < A) {
+      const Line L = lines[j];
+      const bool fl = L.flags & LINE_FLAT;
+      const float xmin = (float)((fl ? Y : X) - h);
+      const float ymin = (float)((fl ? X : Y) - h);
+      // backprojection.fut:50-54
+      const float px = (xmin + 0.5f) - L.fc;
+      const float py = (ymin + 0.5f) - L.fs;
+      const float rho = L.cost * px + L.sint * py;
+      const float sf = ceilf((rho - rho0) / drho);
+      const float xbase = ymin * L.slope;
+      float lk[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        // backprojection.fut:56-64
+        const float sp = sf + (float)k;
+        const float rr = sp * drho + rho0;
+        const float xbot = xbase + (rr / L.cost);
+        const float xtop = xbot + L.slope;
+        const float maxx = fmaxf(xbot, xtop), minx = fminf(xbot, xtop);
+        const float lo = fmaxf(maxx - xmin, 0.0f);
+        const float hi = fmaxf((xmin + 1.0f) - minx, 0.0f);
+        const float diff = maxx - minx;
+        const float m = fminf(lo, hi);
+        const float p = (m == 0.0f) ? 0.0f : m / diff;
+        const float l = fminf(p, 1.0f) * L.lbase;
+        const int si = (int)sp;
+        lk[k] = (si >= 0 && si < R && l != 0.0f) ? l : nan;
+      }
+      v0 = lk[0];
+      v1 = lk[1];
+      const long col = (long)j * R + (long)(int)sf;
+      if (j == 0) first[sl * 64 + lane] = (int32_t)(col - R);
+      step = j == 0 ? R : col - prev;
+      prev = col;
+      over = over || step < -32768 || step > 32767;
+    } else if (j == 0) {
+      first[sl * 64 + lane] = 0;
+    }
+    word |= (uint32_t)(uint16_t)(int16_t)step << (16 * (j & 1));
+    if (j & 1) {
+      ctrl[(o / 2 + j / 2) * 64 + lane] = word;
+      word = 0;
+    }
+    val[(o + j) * 64 + lane] = make_float2(v0, v1);
+  }
+  if (over) atomicOr(bad, 1u);
+}
+
+// y = B r on the row slots of B: one wave per slice of 64 pixels, records in line-table order
+// (the CSR order: bitwise equal to variants 1, 7 and 9).  r is the input with its rows in
+// line-table order; gathers through a buffer descriptor over it (out-of-range columns read 0).
+__global__ __launch_bounds__(256) void k_spmv_rsb(const uint32_t* __restrict__ ctrl, const float2* __restrict__ val,
+                                                  const int32_t* __restrict__ first, long slices, int A8, int N,
+                                                  long ncols, long nrows, const float* __restrict__ in,
+                                                  float* __restrict__ y) {
+  const long sl = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (sl >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const int nxb = (N + 63) / 64;
+  const int Y = (int)(sl / nxb), X = (int)(sl - (long)Y * nxb) * 64 + lane;
+  const float* src = in + (size_t)blockIdx.y * ncols;
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)src, 0, (int)(ncols * 4), 0x00020000u);
+  const long o = sl * A8;
+  const uint32_t* cp = ctrl + (o / 2) * 64 + lane;
+  const vf2* vp = (const vf2*)(val + o * 64 + lane);
+  int cur = first[sl * 64 + lane];
+  float acc = 0.0f;
+  uint32_t w[4];
+  vf2 v[8];
+#pragma unroll
+  for (int hh = 0; hh < 4; ++hh) w[hh] = __builtin_nontemporal_load(cp + 64 * hh);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(vp + 64 * u);
+  for (int j = 0; j < A8; j += 8) {
+    float g0[8], g1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t ww = w[u >> 1];
+      cur += (u & 1) ? ((int)ww >> 16) : (int)(int16_t)(ww & 0xffffu);
+      g0[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, cur << 2, 0, 0));
+      g1[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (cur + 1) << 2, 0, 0));
+    }
+    const bool more = j + 8 < A8;  // uniform
+    uint32_t wn[4];
+    vf2 vn[8];
+    __builtin_amdgcn_sched_barrier(0);
+    if (more) {
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh) wn[hh] = __builtin_nontemporal_load(cp + 64 * ((j + 8) / 2 + hh));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vn[u] = __builtin_nontemporal_load(vp + 64 * (j + 8 + u));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (v[u].x == v[u].x) acc = fmaf(v[u].x, g0[u], acc);
+      if (v[u].y == v[u].y) acc = fmaf(v[u].y, g1[u], acc);
+    }
+    if (more) {
+#pragma unroll
+      for (int hh = 0; hh < 4; ++hh) w[hh] = wn[hh];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = vn[u];
+    }
+  }
+  if (X < N) y[(size_t)blockIdx.y * nrows + (size_t)Y * N + X] = acc;
+}
+
 static void launch_rows(const Geom& G, int kind, bool fill, int32_t* counts, const int64_t* rowptr,
                         int32_t* cols, float* vals, hipStream_t s) {
   const long rows = kind == SK_MAT_FP ? (long)G.A * G.R : (long)G.N * G.N;
@@ -778,6 +1119,12 @@ static void mat_free(sk_mat* m) {
   if (m->sell_d16) (void)hipFree(m->sell_d16);
   if (m->sell_first) (void)hipFree(m->sell_first);
   if (m->line_pos) (void)hipFree(m->line_pos);
+  if (m->rs_off) (void)hipFree(m->rs_off);
+  if (m->rs_ctrl) (void)hipFree(m->rs_ctrl);
+  if (m->rs_val) (void)hipFree(m->rs_val);
+  if (m->rs_first) (void)hipFree(m->rs_first);
+  if (m->rs_lpos) (void)hipFree(m->rs_lpos);
+  if (m->rs_lines) (void)hipFree(m->rs_lines);
   delete m;
 }
 
@@ -882,6 +1229,117 @@ static int build_d16(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
   return SK_OK;
 }
 
+// The row slots of variant 10 (SK_MAT_FP), built from the geometry's line table with the
+// expressions of the CSR fill (so every value is bitwise the CSR's).  With `required` false it
+// is built only if it takes at most half the free device memory.  Synchronizes `s`.
+static int build_rs(Ctx* c, sk_mat* m, bool required, hipStream_t s) {
+  if (m->rs_val) return SK_OK;
+  auto unavailable = [&](const char* why) {
+    m->rs_failed = true;
+    return required ? fail(c, SK_EINVAL, "SpMV variant 10 unavailable: %s", why) : SK_OK;
+  };
+  if (m->kind == SK_MAT_ADJ || !m->rs_lines) return unavailable("row slots exist for A and B only");
+  if (m->rs_failed) return unavailable("a step does not fit its field (or no memory)");
+  if (m->kind == SK_MAT_BP) {
+    // B: a fixed A8 records (lines) per slice of 64 pixels, no count pass
+    if (m->ncand != 2) return unavailable("B's row slots hold two candidates per line (deltarho in [0.71, 1.41))");
+    const int nxb = (m->N + 63) / 64;
+    const long slices = (long)m->N * nxb;
+    const long A8 = (m->A + 7) & ~7;
+    size_t free_b = 0, total_b = 0;
+    SK_HIP(c, hipMemGetInfo(&free_b, &total_b));
+    const size_t recs = (size_t)slices * A8 * 64;
+    const size_t need = recs * (sizeof(float2) + 2) + sizeof(int32_t) * 64 * (size_t)slices;
+    if (!required && need > free_b / 2) return unavailable("no room for the copy");
+    unsigned* bad = nullptr;
+    hipError_t e = hipMalloc(&m->rs_val, sizeof(float2) * recs);
+    if (e == hipSuccess) e = hipMalloc(&m->rs_ctrl, sizeof(uint32_t) * (recs / 2));
+    if (e == hipSuccess) e = hipMalloc(&m->rs_first, sizeof(int32_t) * 64 * (size_t)slices);
+    if (e == hipSuccess) e = hipMalloc(&bad, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(unsigned), s);
+    unsigned hbad = 1;
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_rsb_fill, dim3((unsigned)((slices + 3) / 4)), dim3(SP_BLOCK), 0, s, m->rs_lines, m->A,
+                         (int)A8, m->R, m->N, m->rho0, m->drho, m->rs_ctrl, m->rs_val, m->rs_first, bad);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (bad) (void)hipFree(bad);
+    if (e != hipSuccess || hbad) {
+      if (e != hipSuccess) (void)hipGetLastError();
+      for (void* q : {(void*)m->rs_val, (void*)m->rs_ctrl, (void*)m->rs_first})
+        if (q) (void)hipFree(q);
+      m->rs_val = nullptr;
+      m->rs_ctrl = nullptr;
+      m->rs_first = nullptr;
+      if (e != hipSuccess && required) return hip_fail(c, e, "row-slot copy");
+      return unavailable(e != hipSuccess ? "no memory for the copy" : "a column step does not fit 16 bits");
+    }
+    return SK_OK;
+  }
+  const int nbb = (m->R + 63) / 64;
+  const long slices = (long)m->A * nbb;
+  const unsigned blocks = (unsigned)((slices + 3) / 4);
+  int32_t *lo = nullptr, *len = nullptr;
+  SK_HIP(c, hipMalloc(&lo, sizeof(int32_t) * (size_t)slices));
+  hipError_t e = hipMalloc(&len, sizeof(int32_t) * (size_t)slices);
+  if (e == hipSuccess && !m->rs_off) e = hipMalloc(&m->rs_off, sizeof(int64_t) * (size_t)(slices + 1));
+  if (e != hipSuccess) {
+    (void)hipFree(lo);
+    if (len) (void)hipFree(len);
+    return hip_fail(c, e, "row-slot index allocation");
+  }
+  hipLaunchKernelGGL(k_rs_count, dim3(blocks), dim3(SP_BLOCK), 0, s, m->rs_lines, m->rs_lpos, m->A, m->R, m->N,
+                     m->rho0, m->drho, nbb, lo, len);
+  int rc = launched(c, "row-slot count kernel");
+  int64_t total = 0;
+  if (!rc) rc = scan_counts(c, len, slices, m->rs_off, &total, s);
+  (void)hipFree(len);
+  if (rc) {
+    (void)hipFree(lo);
+    return rc;
+  }
+  size_t free_b = 0, total_b = 0;
+  e = hipMemGetInfo(&free_b, &total_b);
+  const size_t need = (size_t)total * 64 * (sizeof(float2) + 1) + sizeof(int32_t) * 64 * (size_t)slices;
+  if (e != hipSuccess || (!required && need > free_b / 2)) {
+    (void)hipFree(lo);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemGetInfo");
+    return unavailable("no room for the copy");
+  }
+  unsigned* bad = nullptr;
+  e = hipMalloc(&m->rs_val, sizeof(float2) * 64 * (size_t)std::max<int64_t>(total, 4));
+  if (e == hipSuccess) e = hipMalloc(&m->rs_ctrl, sizeof(uint32_t) * 16 * (size_t)std::max<int64_t>(total, 4));
+  if (e == hipSuccess) e = hipMalloc(&m->rs_first, sizeof(int32_t) * 64 * (size_t)slices);
+  if (e == hipSuccess) e = hipMalloc(&bad, sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(unsigned), s);
+  unsigned hbad = 1;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rs_fill, dim3(blocks), dim3(SP_BLOCK), 0, s, m->rs_lines, m->rs_lpos, m->A, m->R, m->N,
+                       m->rho0, m->drho, nbb, lo, m->rs_off, m->rs_ctrl, m->rs_val, m->rs_first, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(lo);
+  if (bad) (void)hipFree(bad);
+  if (e != hipSuccess || hbad) {
+    if (e != hipSuccess) (void)hipGetLastError();
+    for (void* p : {(void*)m->rs_val, (void*)m->rs_ctrl, (void*)m->rs_first})
+      if (p) (void)hipFree(p);
+    m->rs_val = nullptr;
+    m->rs_ctrl = nullptr;
+    m->rs_first = nullptr;
+    if (e != hipSuccess && required) return hip_fail(c, e, "row-slot copy");
+    return unavailable(e != hipSuccess ? "no memory for the copy" : "a row step does not fit its 4-bit field");
+  }
+  if (getenv("SYNKRO_VERBOSE"))
+    fprintf(stderr, "sk_spmv: row slots, %lld records of 64 lanes for %lld nonzeros\n", (long long)total,
+            (long long)m->nnz);
+  return SK_OK;
+}
+
 extern "C" {
 
 int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stream) {
@@ -909,6 +1367,14 @@ int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stre
     for (const Line& L : G.lines) st[(size_t)L.orig] = (L.flags & LINE_FLAT) ? 0 : 1;
     e = hipMalloc(&m->steep, st.size());
     if (e == hipSuccess) e = hipMemcpy(m->steep, st.data(), st.size(), hipMemcpyHostToDevice);
+    std::vector<int32_t> lp((size_t)G.A, 0);
+    for (size_t j = 0; j < G.lines.size(); ++j) lp[(size_t)G.lines[j].orig] = (int32_t)j;
+    if (e == hipSuccess) e = hipMalloc(&m->rs_lpos, sizeof(int32_t) * lp.size());
+    if (e == hipSuccess) e = hipMemcpy(m->rs_lpos, lp.data(), sizeof(int32_t) * lp.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&m->rs_lines, sizeof(Line) * (size_t)G.A);
+    if (e == hipSuccess) e = hipMemcpy(m->rs_lines, G.lines.data(), sizeof(Line) * (size_t)G.A, hipMemcpyHostToDevice);
+    m->rho0 = G.rho0;
+    m->drho = G.drho;
     if (e != hipSuccess) {
       mat_free(m);
       return hip_fail(c, e, "steep-angle table");
@@ -921,6 +1387,12 @@ int sk_sparse_build(sk_ctx* ctx, sk_geom* g, int which, sk_mat** out, void* stre
     }
     e = hipMalloc(&m->line_pos, sizeof(int32_t) * t.size());
     if (e == hipSuccess) e = hipMemcpy(m->line_pos, t.data(), sizeof(int32_t) * t.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && which == SK_MAT_BP) e = hipMalloc(&m->rs_lines, sizeof(Line) * (size_t)G.A);
+    if (e == hipSuccess && which == SK_MAT_BP)
+      e = hipMemcpy(m->rs_lines, G.lines.data(), sizeof(Line) * (size_t)G.A, hipMemcpyHostToDevice);
+    m->rho0 = G.rho0;
+    m->drho = G.drho;
+    m->ncand = G.ncand;
     if (e != hipSuccess) {
       mat_free(m);
       return hip_fail(c, e, "line-order table");
@@ -996,8 +1468,9 @@ int sk_sparse_count(sk_ctx* ctx, sk_geom* g, int which, int64_t* nnz_out, void* 
 int sk_mat_spmv_variant(const sk_mat* m) { return m ? m->spmv_variant : -1; }
 
 int sk_mat_set_spmv_variant(sk_mat* m, int variant) {
-  if (!m || variant < -1 || variant > 9 || variant == 5 || variant == 6) return SK_EINVAL;
+  if (!m || variant < -1 || variant > 10 || variant == 5 || variant == 6) return SK_EINVAL;
   if ((variant == 3 || variant == 4 || variant == 8) && !(m->kind == SK_MAT_FP && m->steep)) return SK_EINVAL;
+  if (variant == 10 && m->kind == SK_MAT_ADJ) return SK_EINVAL;
   m->spmv_variant = variant;
   return SK_OK;
 }
@@ -1060,7 +1533,29 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
   const double invN = m->N > 0 ? 1.0 / (double)m->N : 0.0;
   auto run = [&](int v) -> int {
     dim3 grid((unsigned)((m->rows + 3) / 4), batch);
-    if (v == 9) {
+    if (v == 10 && m->kind == SK_MAT_BP) {
+      int rc = build_rs(c, const_cast<sk_mat*>(m), true, s);
+      if (rc) return rc;
+      void* ws = nullptr;  // the input's rows in line-table order
+      rc = scratch(c, sizeof(float) * (size_t)batch * m->cols, s, &ws);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_rows_permute, dim3((unsigned)((long)batch * m->A)), dim3(256), 0, s, d_in,
+                         m->line_orig, m->A, m->R, (float*)ws);
+      const long slices = (long)m->N * ((m->N + 63) / 64);
+      hipLaunchKernelGGL(k_spmv_rsb, dim3((unsigned)((slices + 3) / 4), batch), dim3(256), 0, s, m->rs_ctrl,
+                         m->rs_val, m->rs_first, slices, (m->A + 7) & ~7, m->N, (long)m->cols, (long)m->rows,
+                         (const float*)ws, d_out);
+    } else if (v == 10) {
+      int rc = build_rs(c, const_cast<sk_mat*>(m), true, s);
+      if (rc) return rc;
+      rc = need_xT();
+      if (rc) return rc;
+      const int nbb = (m->R + 63) / 64;
+      const long slices = (long)m->A * nbb;
+      hipLaunchKernelGGL(k_spmv_rs, dim3((unsigned)((slices + 3) / 4), batch), dim3(256), 0, s, m->rs_off,
+                         m->rs_ctrl, m->rs_val, m->rs_first, m->steep, slices, nbb, m->R, m->N, (long)m->cols,
+                         (long)m->rows, d_in, xT, d_out);
+    } else if (v == 9) {
       int rc = build_d16(c, const_cast<sk_mat*>(m), true, s);
       if (rc) return rc;
       const long slices = (m->rows + 63) / 64;
@@ -1120,27 +1615,32 @@ int sk_spmv(sk_ctx* ctx, const sk_mat* m, int batch, const float* d_in, float* d
     // SELL-64 (7) is a candidate when its copy fits in half the free memory (built here,
     // outside the timed region).
     // Its 16-bit-step form (9) likewise, when every step fits.
+    // The row slots (10) come first: where memory allows only one copy beside the CSR arrays
+    // (2048^2: about 75-90 GB each), it is theirs.
     {
-      int rc = build_sell(c, const_cast<sk_mat*>(m), false, s);
+      int rc = m->kind == SK_MAT_ADJ ? SK_OK : build_rs(c, const_cast<sk_mat*>(m), false, s);
+      if (!rc) rc = build_sell(c, const_cast<sk_mat*>(m), false, s);
       if (!rc && m->sell_col) rc = build_d16(c, const_cast<sk_mat*>(m), false, s);
       if (rc) return rc;
     }
     const bool sell = m->sell_col != nullptr;
     const bool d16 = m->sell_d16 != nullptr;
+    const bool rs = m->rs_val != nullptr;
     // (8 loads in flight per lane instead of 4 measured no faster: 45.8 vs 43.9 ms for A,
     // 63.5 vs 62.6 ms for B at 2048^2.)
-    int cand[7] = {0, 2, 3, 4};
+    int cand[8] = {0, 2, 3, 4};
     int NV = fpT ? 4 : 2;
     if (sell) cand[NV++] = 7;
     if (sell && fpT) cand[NV++] = 8;
     if (d16) cand[NV++] = 9;
-    hipEvent_t e[8];
+    if (rs) cand[NV++] = 10;
+    hipEvent_t e[9];
     for (int v = 0; v <= NV; ++v) SK_HIP(c, hipEventCreate(&e[v]));
     if (fpT) {
       int rc = need_xT();  // outside the timed region of variants 3, 4 (as for every later call)
       if (rc) return rc;
     }
-    if (d16 && m->line_orig) {  // variant 9's permuted input: its scratch grown before timing
+    if ((d16 || rs) && m->line_orig) {  // variants 9, 10: the permuted input's scratch grown before timing
       void* ws = nullptr;
       int rc = scratch(c, sizeof(float) * (size_t)batch * m->cols, s, &ws);
       if (rc) return rc;

@@ -322,7 +322,9 @@ class Matrix:
         SELL-64 copy of the matrix (built on first use, about the CSR's size), 8: as 7 with the
         flat rays' gathers from x^T (A only), 9: as 7 with 16-bit column steps (6 bytes per
         entry; ValueError on use when a row's consecutive columns are more than 32767 apart);
-        -1: time the candidates on next use and keep the fastest."""
+        10: row slots, a copy whose 64-row slices walk in step (A: frame rows, B: lines), so a
+        wave's gathers fall on one frame row or one line (A and B only; B needs deltarho in
+        [0.71, 1.41)); -1: time the candidates on next use and keep the fastest."""
         nat.check(nat.lib().sk_mat_set_spmv_variant(self.handle, int(variant)), self.geom.ctx.handle)
 
     def spmv(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -5,8 +5,8 @@
   (forwardprojection.fut:71-84, backprojection.fut:82-93), with every kernel variant -- bitwise
   in exact mode, rel-L2 <= 1e-5 in fast mode;
 * the explicit-matrix SpMV at the same geometry (forwardprojection_sparse.fut:26-40,
-  backprojection_sparse.fut:26-37), on the tuned SELL-64 kernel with 16-bit column steps: nnz
-  = 1.09e10 > 2^31, so the 64-bit slice offsets are exercised;
+  backprojection_sparse.fut:26-37), on the SELL-64 kernel with 16-bit column steps and the
+  row-slot kernel: nnz = 1.09e10 > 2^31, so the 64-bit slice offsets are exercised;
 * the batched-SIRT problem set (64 x 256^2/180, 96 x 512^2/360, 32 x 1024^2/720 and a SIRT3D
   stack of 512 slices of 512^2/512) through one batched launch sequence: bitwise equal to the
   groups run one by one, and one problem of each size against the oracle's SIRT
@@ -90,26 +90,27 @@ def test_c2_dense_whole_image_fast(ctx, c2, fv, bv):
         assert zero_ok
 
 
-@pytest.mark.parametrize("kind", ["fp", "bp"])
-def test_c2_sparse_spmv_tuned_variant(ctx, c2, kind):
+@pytest.mark.parametrize("kind,variant", [("fp", 9), ("fp", 10), ("bp", 9), ("bp", 10)])
+def test_c2_sparse_spmv_tuned_variant(ctx, c2, kind, variant):
     """The explicit matrix at 2048^2 (A: 5.9 M rows; B: 4.2 M rows; 1.09e10 entries each, so
-    slice offsets and row pointers run past 2^31) on SpMV variant 9 (SELL-64, 16-bit column
-    steps): the whole output against the oracle and against the dense kernel."""
+    slice offsets and row pointers run past 2^31) on the SpMV kernels the autotuner picks
+    there: variant 9 (SELL-64, 16-bit column steps) and variant 10 (row slots): the whole output
+    against the oracle and against the dense kernel."""
     geo = ctx.geometry(c2["ang"], RZ, 1.0, R, N)
     inp = torch.from_numpy(c2["x"] if kind == "fp" else c2["y"]).cuda()
     dense = (geo.fp(inp) if kind == "fp" else geo.bp(inp)).cpu().numpy().ravel()
     m = geo.sparse(kind)
     try:
         assert m.nnz > 2 ** 31
-        m.set_spmv_variant(9)
+        m.set_spmv_variant(variant)
         got = m.spmv(inp).cpu().numpy().ravel()
-        assert m.spmv_variant == 9
+        assert m.spmv_variant == variant
     finally:
         geo.drop_sparse()
         torch.cuda.empty_cache()
     ref = c2["fx"] if kind == "fp" else c2["by"]
     e_ref, e_dense = rel_l2(got, ref), rel_l2(got, dense)
-    print(f"sparse {kind} variant 9 nnz {m.nnz}: vs oracle {e_ref:.2e}, vs dense {e_dense:.2e}")
+    print(f"sparse {kind} variant {variant} nnz {m.nnz}: vs oracle {e_ref:.2e}, vs dense {e_dense:.2e}")
     assert e_ref <= TOL and e_dense <= TOL
 
 
@@ -119,7 +120,7 @@ C3 = [(256, 180, 64), (512, 360, 96), (1024, 720, 32), (512, 512, 512)]
 def test_c3_batched_problem_set(ctx):
     """The batched problem set, 3 iterations: batched == per-group SIRT bitwise for all 704
     images (each geometry on its autotuned kernels), and problem 0 of each group within 1e-5
-    of the oracle's SIRT (the 512-slice stack through oracle.sirt3d's slice rule)."""
+    of the oracle's SIRT."""
     from synkrotomo_amd.autotune import tuned
     from synkrotomo_amd.projector import centred_bins, sirt_batched
     iters = 3

@@ -50,13 +50,16 @@ def test_spmv_matches_dense_reference(ctx, name, N, ang, R, rz, dr):
     x = rng.random(N * N).astype(np.float32)
     y = rng.random(ang.size * R).astype(np.float32)
     geo = ctx.geometry(ang, rz, dr, R, N)
-    for variant in (0, 1, 2, 9):
-        for kind in ("fp", "bp"):
-            geo.sparse(kind).set_spmv_variant(variant)
+    ref_a, ref_b = oracle.fp(ang, rz, dr, R, x), oracle.bp(ang, rz, dr, N, y)
+    for variant in (0, 1, 2, 9, 10):
+        geo.sparse("fp").set_spmv_variant(variant)
         ax = geo.sparse("fp").spmv(_dev(x)).cpu().numpy()
+        assert rel_l2(ax, ref_a) <= TOL
+        if variant == 10 and geo.ncand != 2:  # B's row slots hold two candidates per line
+            continue
+        geo.sparse("bp").set_spmv_variant(variant)
         by = geo.sparse("bp").spmv(_dev(y)).cpu().numpy()
-        assert rel_l2(ax, oracle.fp(ang, rz, dr, R, x)) <= TOL
-        assert rel_l2(by, oracle.bp(ang, rz, dr, N, y)) <= TOL
+        assert rel_l2(by, ref_b) <= TOL
     geo.drop_sparse()
 
 
@@ -65,22 +68,30 @@ def test_spmv_variants_bitwise(ctx, name, N, ang, R, rz, dr):
     """Every SpMV kernel that sums a row in the same order gives the same bits, batched too:
     wave per row 0 == 3 (A: steep rays gather from x^T); 16-byte streams 2 == 4 (x^T); lane
     per row 1 == 7 (SELL-64 copy) == 8 (A: SELL-64, flat rays gather from x^T) == 9 (SELL-64
-    with 16-bit column steps).  B rejects 3, 4 and 8; 5, 6 and 10 do not exist."""
+    with 16-bit column steps) == 10 (row slots: A per frame row, gathers from the frame image; B
+    per line, where its two candidates need deltarho in [0.71, 1.41), else ValueError on use).
+    B rejects 3, 4 and 8; 5 and 6 do not exist."""
     rng = np.random.default_rng(13)
     geo = ctx.geometry(ang, rz, dr, R, N)
     for kind, ncols in (("fp", N * N), ("bp", ang.size * R)):
         x = _dev(rng.random((3, ncols)).astype(np.float32))
         M = geo.sparse(kind)
-        variants = (0, 1, 2, 3, 4, 7, 8, 9) if kind == "fp" else (0, 1, 2, 7, 9)
+        slots = kind == "fp" or geo.ncand == 2
+        variants = (0, 1, 2, 3, 4, 7, 8, 9, 10) if kind == "fp" else (0, 1, 2, 7, 9) + ((10,) if slots else ())
         out = {}
         for v in variants:
             M.set_spmv_variant(v)
             assert M.spmv_variant == v
             out[v] = M.spmv(x).cpu().numpy()
-        same = [(1, 7), (1, 9)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else [])
+        same = [(1, 7), (1, 9)] + ([(0, 3), (2, 4), (1, 8)] if kind == "fp" else []) + ([(1, 10)] if slots else [])
         for a, b in same:
             np.testing.assert_array_equal(out[b], out[a], err_msg=f"{kind}: variant {b} vs {a}")
-        for v in ((3, 4, 5, 6, 8, 10) if kind == "bp" else (5, 6, 10)):
+        if not slots:
+            M.set_spmv_variant(10)
+            with pytest.raises(ValueError):
+                M.spmv(x)
+            M.set_spmv_variant(1)
+        for v in ((3, 4, 5, 6, 8, 11) if kind == "bp" else (5, 6, 11)):
             with pytest.raises(ValueError):
                 M.set_spmv_variant(v)
     geo.drop_sparse()
