@@ -530,13 +530,20 @@ __global__ __launch_bounds__(256) void k_wgrad32_mfma(int H, int W, int B, const
   for (int o = threadIdx.x; o < WG_NOUT; o += 256) partials[(size_t)blockIdx.x * WG_NOUT + o] = red[o];
 }
 
+// out[o] = sum over blocks b of partials[b][o] in a fixed order (deterministic): a workgroup
+// takes 64 consecutive outputs (coalesced rows of the partials), wave w sums the blocks
+// b = w, w + 4, ... in increasing order, then the four sums are added in wave order.
 __global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partials, int nblocks,
                                                          int nout, float* __restrict__ out) {
-  const int o = blockIdx.x * 256 + threadIdx.x;
-  if (o >= nout) return;
+  __shared__ float sh[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int o = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * nout + o];
-  out[o] = s;
+  if (o < nout)
+    for (int b = w; b < nblocks; b += 4) s += partials[(size_t)b * nout + o];
+  sh[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && o < nout) out[o] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
 }
 
 // ---------------------------------------------------------------------- phantoms
@@ -729,7 +736,7 @@ int sk_conv_wgrad(sk_ctx* ctx, int kind, int B, int H, int W, const void* g, con
     hipLaunchKernelGGL((k_wgrad<NCH, 2>), dim3(nblocks), dim3(256), 0, s, H, W, B, g, Mk, x, part);
   rc = launched(c, "wgrad kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((nout + 255) / 256), dim3(256), 0, s, part, nblocks, nout, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nout + 63) / 64), dim3(256), 0, s, part, nblocks, nout, out);
   return launched(c, "partials reduction kernel");
 }
 

@@ -98,6 +98,22 @@ def train(args) -> int:
         if args.stage_weight > 0:
             for xs in stages[:-1]:
                 loss = loss + args.stage_weight * torch.mean((xs - x_true) ** 2)
+        if args.tail_steps > 0 and args.tail_weight > 0:
+            # the data residual after `tail_steps` relaxed SIRT steps from the last stage, above
+            # 1.1x the phantom's noise floor: the tail is affine in the stage output, so this
+            # rewards outputs whose remaining error the tail removes fastest
+            from .autograd import project
+            xt = x
+            for _ in range(args.tail_steps):
+                xt = xt + args.tail_relax * (model.sirt_step(xt, b) - xt)
+            bb = b.reshape(b.shape[0], -1)
+            bn = bb.square().sum(1)
+            r2 = (project(xt, geo).reshape(b.shape[0], -1) - bb).square().sum(1) / bn
+            with torch.no_grad():
+                f2 = (clean.reshape(b.shape[0], -1) - bb).square().sum(1) / bn
+            ok = f2 > 0
+            excess = torch.where(ok, torch.relu(r2 - 1.21 * f2) / f2.clamp_min(1e-30), torch.zeros_like(r2))
+            loss = loss + args.tail_weight * torch.mean(excess)
         if args.res_weight > 0 or args.res_excess_weight > 0:
             # data consistency of the final image: ||A x - b|| / ||b|| per image
             from .autograd import project
@@ -138,7 +154,8 @@ def train(args) -> int:
     net.save(args.out, scale=scale, I0=I0, stages=args.stages, N=N, A=A, phantoms=args.phantoms,
              batch=args.batch, epochs=args.epochs, res_weight=args.res_weight,
              stage_weight=args.stage_weight, res_excess_weight=args.res_excess_weight,
-             proj_weight=args.proj_weight, init=os.path.basename(args.init) if args.init else "",
+             proj_weight=args.proj_weight, tail_steps=args.tail_steps, tail_weight=args.tail_weight,
+             init=os.path.basename(args.init) if args.init else "",
              lr=args.lr, train_seconds=train_s)
     print(json.dumps({"saved": args.out, "train_seconds": round(train_s, 1), "scale": scale}), flush=True)
     return 0
@@ -354,6 +371,11 @@ def main(argv=None) -> int:
                    help="weight of the final image's projection error against the noise-free "
                         "data, ||A(x - x_true)||^2 / ||A x_true||^2")
     t.add_argument("--init", default="", help="start from these weights instead of a fresh init")
+    t.add_argument("--tail-steps", type=int, default=0,
+                   help="relaxed SIRT steps after the stages whose final residual enters the loss")
+    t.add_argument("--tail-weight", type=float, default=0.0,
+                   help="weight of that residual's excess over 1.1x the noise floor (relative)")
+    t.add_argument("--tail-relax", type=float, default=1.8, help="relaxation of those steps")
     t.add_argument("--cin", type=int, default=1, choices=[1, 2],
                    help="network input channels: 1 = the image after each SIRT step, 2 = also the "
                         "step's update direction")
