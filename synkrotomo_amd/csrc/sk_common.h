@@ -50,6 +50,10 @@ struct Geom {
   // backprojector intercepts (s*drho + rho0)/cost per table line j and bin s, [A][R] (null when
   // A*R >= 2^31: the kernels then divide)
   float* d_inter = nullptr;
+  // backprojector tile launch orders for 1, 2 and 4 pixels per lane (bp_tile_orders in
+  // sk_geom.hip; null where the hoisted fast path does not apply), one buffer
+  unsigned* d_bp_order = nullptr;
+  size_t bp_order_off[3] = {0, 0, 0};
   // SIRT weights, computed on first use (SIRT.fut:22-25)
   float* d_rowinv = nullptr;  // [A][R]
   float* d_colinv = nullptr;  // [N][N]

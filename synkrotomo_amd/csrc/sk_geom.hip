@@ -6,12 +6,14 @@
 // index the sinogram through Line::orig, so neither the sinogram partition
 // (fix_projections, sirtlib.fut:16-21) nor the output scatter (postprocess_fp,
 // sirtlib.fut:24-25) exists.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
 
 #include "sk_common.h"
+#include "sk_kern.h"
 
 namespace sk {
 
@@ -60,6 +62,36 @@ static float newton_min_slope(int N) {
   const float x = (float)(N / 2 + 4);
   const float half_ulp = 0.5f * (nextafterf(x, INFINITY) - x);
   return half_ulp / 4e-4f;
+}
+
+// Launch orders of the backprojector's tiles (BP_TW x 16*ppt pixels) for ppt = 1, 2, 4, dearest
+// first.  A tile's lines take the exact step (5 VALU per candidate instead of 10) where its
+// frame columns lie in one binade (tile_binade: steep lines over the tile's 16 columns, flat
+// lines over its 64-row band), so the tiles of a launch differ in cost; starting the dear ones
+// first leaves cheap blocks for the end of the launch (longest-first list scheduling).
+static std::vector<unsigned> bp_tile_orders(int N, int n_steep, int A, size_t off[3]) {
+  std::vector<unsigned> all;
+  const int h = N / 2, ntx = (N + BP_TW - 1) / BP_TW;
+  const int ppts[3] = {1, 2, 4};
+  for (int i = 0; i < 3; ++i) {
+    const int TH = 16 * ppts[i], nty = (N + TH - 1) / TH;
+    std::vector<std::pair<long, unsigned>> t;
+    t.reserve((size_t)ntx * nty);
+    for (int ty = 0; ty < nty; ++ty)
+      for (int tx = 0; tx < ntx; ++tx) {
+        const int Y64 = (ty * TH) & ~63;
+        const long cs = tile_binade(tx * BP_TW - h, tx * BP_TW - h + BP_TW - 1) >= 0 ? 61 : 100;
+        const long cf = tile_binade(Y64 - h, Y64 - h + 63) >= 0 ? 61 : 100;
+        t.push_back({-(cs * n_steep + cf * (A - n_steep)), ((unsigned)ty << 16) | (unsigned)tx});
+      }
+    std::stable_sort(t.begin(), t.end(),
+                     [](const std::pair<long, unsigned>& a, const std::pair<long, unsigned>& b) {
+                       return a.first < b.first;
+                     });
+    off[i] = all.size();
+    for (const auto& p : t) all.push_back(p.second);
+  }
+  return all;
 }
 
 // N > 0: also mark the lines of LINE_NEWTON for an N x N image.
@@ -240,9 +272,16 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
     }
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   }
+  if (e == hipSuccess && G.bp_two_cand && N / BP_TW < 65536) {
+    const std::vector<unsigned> ord = bp_tile_orders(N, G.n_steep, A, G.bp_order_off);
+    e = hipMalloc(&G.d_bp_order, sizeof(unsigned) * ord.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(G.d_bp_order, ord.data(), sizeof(unsigned) * ord.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     if (G.d_lines) (void)hipFree(G.d_lines);
     if (G.d_inter) (void)hipFree(G.d_inter);
+    if (G.d_bp_order) (void)hipFree(G.d_bp_order);
     delete g;
     return hip_fail(c, e, "geometry upload");
   }
@@ -258,6 +297,7 @@ void sk_geom_destroy(sk_geom* g) {
     if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
   if (g->g.d_lines) (void)hipFree(g->g.d_lines);
   if (g->g.d_inter) (void)hipFree(g->g.d_inter);
+  if (g->g.d_bp_order) (void)hipFree(g->g.d_bp_order);
   if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
   if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);
   delete g;

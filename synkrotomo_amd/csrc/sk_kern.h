@@ -382,6 +382,10 @@ struct BpParams {
   float* outT;          // BP_SIRT: transposed copy of the updated image
   const float* colinv;  // BP_SIRT: [N][N]
   const float* inter;   // [A][R] (s*drho + rho0)/cost per table line, or null
+  // launch order of the tiles, (tile_y << 16) | tile_x per block in grid order, or null (grid
+  // order): the tiles whose lines take the exact step (tile_binade) are the cheapest, so the
+  // dearer ones go first and the last blocks to start are cheap ones
+  const unsigned* order;
 };
 
 // LDS window width (bins) covering every candidate of a BP_TW x (16*PPT) tile.
@@ -400,10 +404,41 @@ __host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
 struct alignas(16) BpLineLds {
   int w0;
   float ci, si, cst;
-  float cost, lbase;  // the line's (window staging)
+  float rs;      // 1/|slope| on LINE_NEWTON lines (fast path), else 0
+  float dx;      // hoisted fast path: the exact |x_top - x_bot| of every candidate of the tile
+                 // (tile_binade), else 0
+  float cost;    // the line's (window staging)
+  float wscale;  // factor of the staged values: lbase, or lbase/dx when dx > 0
   int orig;
-  float rs;  // 1/|slope| on LINE_NEWTON lines (fast path), else 0
+  int tie;       // hoisted fast path, tile with one binade: the step is a rounding tie (dx = 0)
 };
+
+// Binade [2^e, 2^(e+1)) holding the magnitudes of every x_bot and x_top of a tile's candidates
+// that can have a nonzero length, and of the pixel edges moved by the step (xmin - d,
+// xmin + 1 + d), or -1.  x0..x1: the tile's frame columns (steep lines: image columns; flat
+// lines: image rows).  A candidate with a nonzero length has x_bot, x_top in (xmin - 1,
+// xmin + 2) (backprojection.fut:59-62 with |slope| <= 1); the range [x0 - 4, x1 + 5] also
+// covers every candidate within 3 of its pixel, so one farther away (whose length is 0 in the
+// reference) stays at 0 here.
+__host__ __device__ inline int tile_binade(int x0, int x1) {
+  const int lo = x0 - 4, hi = x1 + 5;
+  if (lo <= 0 && hi >= 0) return -1;
+  const int a = lo > 0 ? lo : -hi, b = lo > 0 ? hi : -lo;  // 1 <= a <= b
+  const int ea = 31 - __builtin_clz((unsigned)a), eb = 31 - __builtin_clz((unsigned)b);
+  return ea == eb ? ea : -1;
+}
+
+// The exact step d = |fl(x_bot + slope) - x_bot| of every x_bot in binade e whose x_top stays
+// in it: x_bot is a multiple of u = 2^(e-23), so fl(x_bot + slope) = x_bot + RN_u(slope), and d
+// = RN_u(|slope|) whatever x_bot is -- unless |slope| lies exactly halfway between two multiples
+// of u, where round-half-even makes d depend on x_bot's last bit (returns 0 then, and for
+// d = 0).  |slope|/u and its rounding are exact (|slope| <= 1, u a power of two, e >= 0).
+__device__ __forceinline__ float binade_step(float slope, int e) {
+  const float t = ldexpf(fabsf(slope), 23 - e);
+  const float ft = floorf(t);
+  if (t - ft == 0.5f) return 0.0f;
+  return ldexpf(rintf(t), e - 23);
+}
 
 // One work item: pixel tile (tile_x, tile_y) of image bz, 16 columns x 16*PPT rows; thread
 // (ty, tx) owns pixels (Y0 + ty + 16k, X0 + tx).  Lines are walked in order in blocks of
@@ -443,6 +478,14 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
     acc_f[k] = 0.0f;
   }
 
+  // hoisted fast path: the tile's binade for steep lines (frame x = image column) and for flat
+  // lines (frame x = image row); -1 where the tile straddles a power of two or zero.  The
+  // rows are taken 64 at a time whatever the tile height, so every tile shape makes the same
+  // choice for a pixel and the variants stay bitwise equal.
+  const int Y64 = Y0 & ~63;
+  const int ebin[2] = {REV ? tile_binade(X0 - h, X0 - h + BP_TW - 1) : -1,
+                       REV ? tile_binade(Y64 - h, Y64 - h + 63) : -1};
+
   for (int l0 = 0; l0 < P.A; l0 += BP_LB) {
     const int nl = min(BP_LB, P.A - l0);
     if ((int)threadIdx.x < nl) {
@@ -467,9 +510,13 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         e.cst = -(fmaf(L.cost, L.fc, L.sint * L.fs) + P.rho0) * idr;
       }
       e.cost = L.cost;
-      e.lbase = L.lbase;
       e.orig = L.orig;
       e.rs = (L.flags & LINE_NEWTON) ? 1.0f / fabsf(L.slope) : 0.0f;
+      const int eb = ebin[fl ? 1 : 0];
+      const bool exd = REV && eb >= 0 && !(L.flags & LINE_DEGEN);
+      e.dx = exd ? binade_step(L.slope, eb) : 0.0f;
+      e.tie = exd && e.dx == 0.0f;
+      e.wscale = e.dx > 0.0f ? L.lbase / e.dx : L.lbase;
       lsh[threadIdx.x] = e;
     }
     __syncthreads();
@@ -498,7 +545,7 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
           const float v = sb[(size_t)lsh[jj].orig * P.R + (in ? sp : 0)];  // in bounds either way
           p = in ? v : 0.0f;
         }
-        if constexpr (FASTREF) p = p * lsh[jj].lbase;
+        if constexpr (FASTREF) p = p * lsh[jj].wscale;
         win[jj * W + (REV ? W - 1 - k : k)] = make_float2(inter, p);
       }
     }
@@ -525,71 +572,91 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         // sign (degenerate or not) form runs; Line::flags >> LINE_RUN_SHIFT is the number of
         // lines from a line to the end of its run.  Each run is walked by a loop whose body
         // has the sign and the degenerate case resolved at compile time (no per-line branch).
-        auto run = [&](auto kind, int jb, int je) {
-          // 0: slope >= 0, 1: slope < 0, 2: degenerate; 3, 4: as 0, 1 on LINE_NEWTON lines
+        // Kinds: 0: slope >= 0, 1: slope < 0, 2: degenerate; 3, 4: as 0, 1 on LINE_NEWTON
+        // lines; 5, 6: as 0, 1 in a tile with one binade (tile_binade), with the exact step.
+        auto line_body = [&](auto kind, int jj) {
           constexpr int KIND = decltype(kind)::value;
-          for (int jj = jb; jj < je; ++jj) {
-            const float slope = P.lines[l0 + jj].slope;
-            const BpLineLds lc = lsh[jj];
-            // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
-            const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
-            // All of the lane's pixels first compute their candidate index and read their
-            // candidates: the LDS reads of all pixels are in flight together.  Bin s sits at
-            // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of
-            // the pixel whose first candidate is s (w0 is uniform: scalar base arithmetic).
-            const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
-            float2 ev[PPT][NC];
-            float xbase[PPT];
+          const float slope = P.lines[l0 + jj].slope;
+          const BpLineLds lc = lsh[jj];
+          // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
+          const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
+          // All of the lane's pixels first compute their candidate index and read their
+          // candidates: the LDS reads of all pixels are in flight together.  Bin s sits at
+          // window position W-1-(s-w0); wrev[-s] is the entry of the last candidate of the
+          // pixel whose first candidate is s (w0 is uniform: scalar base arithmetic).
+          const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
+          float2 ev[PPT][NC];
+          float xbase[PPT], edge[PPT];
 #pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-              // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this
-              // path), so its ceiling lies in [w0, w0 + W - NC]: no clamp.
-              const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
-              const float2* wrow = wrev + neg_ceil_to_int(t);
-              xbase[q] = ym[q] * slope;  // backprojection.fut:55
+          for (int q = 0; q < PPT; ++q) {
+            // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this
+            // path), so its ceiling lies in [w0, w0 + W - NC]: no clamp.
+            const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
+            const float2* wrow = wrev + neg_ceil_to_int(t);
+            xbase[q] = ym[q] * slope;  // backprojection.fut:55
+            // x_top - xmin = x_bot - (xmin - d), resp. xmin + 1 - x_top = (xmin + 1 + d) -
+            // x_bot, exactly: x_bot, x_top and the moved edge are multiples of u in the
+            // tile's binade
+            if constexpr (KIND == 5) edge[q] = xm[q] - lc.dx;
+            if constexpr (KIND == 6) edge[q] = x1[q] + lc.dx;
 #pragma unroll
-              for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
-            }
-            // Non-degenerate lines: max(min(a, b), 0) == med3(a, b, 0): a and b cannot both
-            // be negative when the ray's x-range has positive length; m/diff is then >= 0,
-            // so min(., 1) is a clamp to [0, 1].
-#pragma unroll
-            for (int q = 0; q < PPT; ++q)
-#pragma unroll
-              for (int k = 0; k < NC; ++k) {
-                const float xb = xbase[q] + ev[q][k].x;
-                const float xt = xb + slope;
-                if constexpr (KIND == 0) {
-                  const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
-                  const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
-                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-                } else if constexpr (KIND == 1) {
-                  const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
-                  const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
-                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-                } else if constexpr (KIND == 3 || KIND == 4) {
-                  // 1/d by one Newton step from 1/|slope| (d = |x_top - x_bot| is |slope| to
-                  // within the rounding of x_bot): two full-rate FMAs for the quarter-rate
-                  // v_rcp_f32
-                  const float d = KIND == 3 ? xt - xb : xb - xt;
-                  const float m = KIND == 3 ? __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f)
-                                            : __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
-                  const float r = fmaf(lc.rs, fmaf(-d, lc.rs, 1.0f), lc.rs);
-                  const float f = fminf(fmaxf(m * r, 0.0f), 1.0f);
-                  acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-                } else {
-                  // degenerate line: general form with the m == 0 case, lbase already in e.y
-                  const float l = cand_len<false, W_REF>(xb, slope, 1.0f, xm[q], x1[q], true);
-                  acc[q] = fmaf(l, ev[q][k].y, acc[q]);
-                }
-              }
+            for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
           }
+          // Non-degenerate lines: max(min(a, b), 0) == med3(a, b, 0): a and b cannot both
+          // be negative when the ray's x-range has positive length; m/diff is then >= 0,
+          // so min(., 1) is a clamp to [0, 1].
+#pragma unroll
+          for (int q = 0; q < PPT; ++q)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+              const float xb = xbase[q] + ev[q][k].x;
+              const float xt = xb + slope;
+              if constexpr (KIND == 0) {
+                const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
+                const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
+                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+              } else if constexpr (KIND == 1) {
+                const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
+                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+              } else if constexpr (KIND == 3 || KIND == 4) {
+                // 1/d by one Newton step from 1/|slope| (d = |x_top - x_bot| is |slope| to
+                // within the rounding of x_bot): two full-rate FMAs for the quarter-rate
+                // v_rcp_f32
+                const float d = KIND == 3 ? xt - xb : xb - xt;
+                const float m = KIND == 3 ? __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f)
+                                          : __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                const float r = fmaf(lc.rs, fmaf(-d, lc.rs, 1.0f), lc.rs);
+                const float f = fminf(fmaxf(m * r, 0.0f), 1.0f);
+                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
+              } else if constexpr (KIND == 5 || KIND == 6) {
+                // min(m/d, 1) = min(m, d)/d with d = dx exact and 1/d in the staged value:
+                // max(min(a, b, d), 0) is one v_min3 with the output clamp (d <= 1).  A line
+                // whose step is a rounding tie has dx = 0 and adds 0 here (tie pass below).
+                const float a = KIND == 5 ? xb - edge[q] : xb - xm[q];
+                const float b = KIND == 5 ? x1[q] - xb : edge[q] - xb;
+                const float g = __builtin_amdgcn_fmed3f(fminf(fminf(a, b), lc.dx), 0.0f, 1.0f);
+                acc[q] = fmaf(g, ev[q][k].y, acc[q]);
+              } else {
+                // degenerate line: general form with the m == 0 case, lbase already in e.y
+                const float l = cand_len<false, W_REF>(xb, slope, 1.0f, xm[q], x1[q], true);
+                acc[q] = fmaf(l, ev[q][k].y, acc[q]);
+              }
+            }
         };
+        auto run = [&](auto kind, int jb, int je) {
+          for (int jj = jb; jj < je; ++jj) line_body(kind, jj);
+        };
+        const bool exact_step = ebin[part] >= 0;  // uniform: the tile's binade for this part
         for (int jr = j_begin; jr < j_end;) {  // uniform: scalar loop over the runs
           const int f = P.lines[l0 + jr].flags;
           const int je = min(jr + (f >> LINE_RUN_SHIFT), j_end);
           if (f & LINE_DEGEN)
             run(std::integral_constant<int, 2>{}, jr, je);
+          else if (exact_step && (f & LINE_SLOPE_POS))
+            run(std::integral_constant<int, 5>{}, jr, je);
+          else if (exact_step)
+            run(std::integral_constant<int, 6>{}, jr, je);
           else if ((f & LINE_NEWTON) && (f & LINE_SLOPE_POS))
             run(std::integral_constant<int, 3>{}, jr, je);
           else if (f & LINE_NEWTON)
@@ -599,6 +666,21 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
           else
             run(std::integral_constant<int, 1>{}, jr, je);
           jr = je;
+        }
+        if (exact_step) {
+          // tie pass: the part's lines whose step depends on x_bot's last bit, with the
+          // hardware reciprocal (their staged values carry lbase only)
+          const int lane = threadIdx.x & 63;
+          const bool tie = lane >= j_begin && lane < j_end && lsh[lane < BP_LB ? lane : 0].tie;
+          unsigned long long tm = __ballot(tie);
+          while (tm) {  // uniform
+            const int jj = __builtin_ctzll(tm);
+            tm &= tm - 1;
+            if (P.lines[l0 + jj].flags & LINE_SLOPE_POS)
+              line_body(std::integral_constant<int, 0>{}, jj);
+            else
+              line_body(std::integral_constant<int, 1>{}, jj);
+          }
         }
       }
       if constexpr (!REV) for (int jj = j_begin; jj < j_end; ++jj) {

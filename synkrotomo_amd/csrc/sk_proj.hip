@@ -83,10 +83,16 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
 
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
-__global__ __launch_bounds__(256) void k_bp(BpParams P) {
+__global__ __launch_bounds__(256, NC > 0 ? 8 : 1) void k_bp(BpParams P) {
   extern __shared__ float2 win[];
   __shared__ BpLineLds lsh[BP_LB];
-  bp_item<EXACT, MODE, WEIGHTS, NC, DR1, PPT>(P, blockIdx.x, blockIdx.y, blockIdx.z, win, lsh);
+  int tx = blockIdx.x, ty = blockIdx.y;
+  if (P.order) {
+    const unsigned t = P.order[blockIdx.y * gridDim.x + blockIdx.x];
+    tx = (int)(t & 0xffffu);
+    ty = (int)(t >> 16);
+  }
+  bp_item<EXACT, MODE, WEIGHTS, NC, DR1, PPT>(P, tx, ty, blockIdx.z, win, lsh);
 }
 
 // ------------------------------------------------------------------------ launchers
@@ -181,8 +187,10 @@ int launch_fp_bpt(Ctx* c, const Geom* g, const FpArgs& a, hipStream_t s) {
 }
 
 BpParams bp_params(const Geom* g, const BpArgs& a, int ppt) {
+  const unsigned* order =
+      g->d_bp_order ? g->d_bp_order + g->bp_order_off[ppt == 1 ? 0 : (ppt == 2 ? 1 : 2)] : nullptr;
   return BpParams{g->d_lines, g->A, g->R, g->N, g->ncand, bp_window_bins(ppt, g->drho, g->ncand),
-                  g->n_steep, g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv, g->d_inter};
+                  g->n_steep, g->rho0, g->drho, a.sino, a.out, a.outT, a.colinv, g->d_inter, order};
 }
 
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
