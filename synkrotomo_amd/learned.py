@@ -278,18 +278,62 @@ class LearnedSIRT:
         return outs if all_stages else x
 
     @torch.no_grad()
-    def reconstruct(self, b: torch.Tensor):
-        """Returns (images, fell_back [B] bool)."""
+    def reconstruct(self, b: torch.Tensor, guard: bool = True):
+        """The stages, each correction that yields a non-finite image dropped for its stage.
+        guard: also run ``stages`` plain SIRT steps and return them for every image whose data
+        residual they beat (the equal-cost fallback rule).  Returns (images, fell_back [B])."""
         B = b.shape[0]
         x = torch.zeros((B, self.geom.N, self.geom.N), device=b.device)
         xp = None
         for k in range(self.stages):
             xs = self.stage_step(x, b, k)
-            # both paths start from zero, so plain SIRT's first step is the first stage's (the
-            # kernels are deterministic: the same bits)
-            xp = xs.clone() if k == 0 else self.sirt_step(xp, b)
+            if guard:
+                # both paths start from zero, so plain SIRT's first step is the first stage's
+                # (the kernels are deterministic: the same bits)
+                xp = xs.clone() if k == 0 else self.sirt_step(xp, b)
             xc = self.net(xs, xs - x) if self.net.cin == 2 else self.net(xs)
             bad = ~torch.isfinite(xc).reshape(B, -1).all(dim=1)
             x = torch.where(bad.view(B, 1, 1), xs, xc)
+        if not guard:
+            return x, torch.zeros(B, dtype=torch.bool, device=b.device)
         worse = residual(self.geom, x, b) > residual(self.geom, xp, b)
         return torch.where(worse.view(B, 1, 1).to(x.device), xp, x), worse
+
+    @torch.no_grad()
+    def solve(self, b: torch.Tensor, target: float | torch.Tensor, max_iters: int,
+              check_every: int = 1):
+        """Reconstruct to a data residual: the stages, then SIRT steps relaxed by tail_relax
+        until ||A x - b|| / ||b|| <= target (a number or one per image), each image kept at the
+        first check where it is at or below its target.  The residual judges the output: an
+        image that has not reached its target after ``max_iters`` iterations in all (the stages
+        count as one each) falls back to plain SIRT from zero with the same budget, and the
+        result with the lower residual is returned.
+        Returns (images [B, N, N], iterations [B] int64, fell_back [B] bool)."""
+        if max_iters < self.stages:
+            raise ValueError("max_iters must cover the stages")
+        B = b.shape[0]
+        tgt = torch.as_tensor(target, dtype=torch.float64, device=b.device).expand(B)
+        x, _ = self.reconstruct(b, guard=False)
+        out = x.clone()
+        iters = torch.full((B,), -1, dtype=torch.int64, device=b.device)
+        k = self.stages
+        while True:
+            hit = (iters < 0) & (residual(self.geom, x, b) <= tgt)
+            out[hit] = x[hit]
+            iters[hit] = k
+            if bool((iters >= 0).all()) or k >= max_iters:
+                break
+            n = min(check_every, max_iters - k)
+            self.tail_(x, b, n)
+            k += n
+        fell = iters < 0
+        if bool(fell.any()):
+            idx = fell.nonzero().flatten()
+            xp = torch.zeros((idx.numel(), self.geom.N, self.geom.N), device=b.device)
+            bp = b[idx].contiguous()
+            self.geom.sirt_(xp, bp, max_iters)
+            better = residual(self.geom, xp, bp) < residual(self.geom, x[idx], bp)
+            out[idx] = torch.where(better.view(-1, 1, 1), xp, x[idx])
+            iters[idx] = max_iters
+            fell[idx] = better
+        return out, iters, fell

@@ -19,8 +19,11 @@ floor (as consistent with the data as the true image, within 10%).
 Time to a target = (iterations or stages needed) x (measured time per iteration or stage
 on the GPU, batch of images); errors are the relative image error against the true
 phantom computed in FP64.  The learned time is the measured time of the whole learned
-reconstruction (stages, the finiteness guard, and the fallback rule's plain SIRT steps and
-residuals) plus the plain iterations after it; the stages-only time is reported beside it.
+reconstruction (stages, the finiteness guard, and the equal-cost fallback rule's plain SIRT
+steps and residuals) plus the relaxed SIRT iterations after it; the stages-only time is
+reported beside it, and so is the residual-judged method (``LearnedSIRT.solve``: the stages
+with the finiteness guard, then the iterations to the target, plain SIRT only for an image
+that misses it within the budget).
 """
 from __future__ import annotations
 
@@ -222,7 +225,8 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
             model.tail_(xb, b, 1)
             model.stage(xb, b, 1)
         model.reconstruct(b)  # warm-up
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        model.reconstruct(b, guard=False)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
         e[0].record()
         geo.sirt_(xb, b, 20)
         e[1].record()
@@ -235,11 +239,16 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
         e[3].record()
         model.tail_(xb, b, 20)
         e[4].record()
+        # the stages with the finiteness guard only (the residual-judged method: the data
+        # residual decides, plain SIRT is the fallback if the target is not reached)
+        model.reconstruct(b, guard=False)
+        e[5].record()
         torch.cuda.synchronize()
     t_iter = e[0].elapsed_time(e[1]) / 20 / batch * 1e-3
     t_stage = e[1].elapsed_time(e[2]) / 5 / batch * 1e-3
     t_recon = e[2].elapsed_time(e[3]) / batch * 1e-3
     t_tail = e[3].elapsed_time(e[4]) / 20 / batch * 1e-3
+    t_recon_ng = e[4].elapsed_time(e[5]) / batch * 1e-3
 
     rows = []
     for start in range(0, n_test, batch):
@@ -254,10 +263,18 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
             xl, fell = model.reconstruct(b)
             kl, res_l, err_l = _track(lambda z: model.tail_(z, b, 1), xl, geo, b, x_true, 0,
                                       max_iters - stages, dense, stride, 1e-2)
+            if bool(fell.any()):  # the unguarded stages differ from the guarded output there
+                xn, _ = model.reconstruct(b, guard=False)
+                kn, res_n, err_n = _track(lambda z: model.tail_(z, b, 1), xn, geo, b, x_true, 0,
+                                          max_iters - stages, dense, stride, 1e-2)
+            else:
+                kn, res_n, err_n = kl, res_l, err_l
         k5 = int(np.searchsorted(kp, stages))
         for tname, target in (("1e-2", np.full(B, 1e-2)), ("1.1x_floor", 1.1 * floor)):
             hp = _first_hit(kp, res_p, target)
             hl = _first_hit(kl, res_l, target)
+            hn = _first_hit(kn, res_n, target)
+            jn = np.array([int(np.searchsorted(kn, h)) if h >= 0 else -1 for h in hn])
             # image error plain SIRT has when it reaches the target, and the first learned
             # state (after the stages, then after tail iterations) at or below it
             ip = np.array([int(np.searchsorted(kp, h)) if h >= 0 else -1 for h in hp])
@@ -274,12 +291,15 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                              "learned_res_after_stages": float(res_l[0, i]),
                              "learned_err_after_stages": float(err_l[0, i]),
                              "plain_err_at_stages": float(err_p[min(k5, len(kp) - 1), i]),
-                             "learned_extra_iters_to_plain_err": int(he[i]) if hp[i] >= 0 else -1})
+                             "learned_extra_iters_to_plain_err": int(he[i]) if hp[i] >= 0 else -1,
+                             "judged_extra_iters": int(hn[i]),
+                             "judged_err": float(err_n[jn[i], i]) if hn[i] >= 0 else None})
         if verbose:
             print(json.dumps({"done": start + B}), flush=True)
 
     out = {"t_iter_ms": round(t_iter * 1e3, 4), "t_tail_iter_ms": round(t_tail * 1e3, 4),
            "t_stage_ms": round(t_stage * 1e3, 4), "t_reconstruct_ms": round(t_recon * 1e3, 4),
+           "t_stages_unguarded_ms": round(t_recon_ng * 1e3, 4),
            "scale": scale, "tail_relax": tail_relax,
            "stage_relax": [round(float(v), 4) for v in net.relax.detach().cpu()[1:stages]],
            "n_test": n_test, "max_iters": max_iters, "dense_iters": dense, "stride": stride,
@@ -317,6 +337,19 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
                         [r["learned_extra_iters_to_plain_err"] for r in eq])),
                     "median_learned_ms": round(float(np.median(tle)) * 1e3, 3),
                     "speedup_median": round(float(np.median(tpe) / np.median(tle)), 2)}
+            # residual-judged method (LearnedSIRT.solve): the stages with the finiteness guard,
+            # then tail steps to the target; plain SIRT only for an image that misses it
+            jb = [r for r in rs if r["plain_iters"] >= 0 and r["judged_extra_iters"] >= 0]
+            if jb:
+                tpj = np.array([r["plain_iters"] * t_iter for r in jb])
+                tlj = np.array([t_recon_ng + r["judged_extra_iters"] * t_tail for r in jb])
+                summary["residual_judged"] = {
+                    "of": len(jb), "missed_target": sum(r["judged_extra_iters"] < 0 for r in rs),
+                    "median_learned_ms": round(float(np.median(tlj)) * 1e3, 3),
+                    "speedup_median": round(float(np.median(tpj) / np.median(tlj)), 2),
+                    "median_learned_extra_iters": float(np.median([r["judged_extra_iters"] for r in jb])),
+                    "median_learned_err": round(float(np.median([r["judged_err"] for r in jb])), 5),
+                    "median_plain_err": round(float(np.median([r["plain_err"] for r in jb])), 5)}
         out[tname] = summary
     out["fell_back"] = sum(r["fell_back"] for r in rows if r["target"] == "1e-2")
     rs = [r for r in rows if r["target"] == "1e-2"]

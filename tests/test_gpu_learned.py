@@ -234,6 +234,44 @@ def test_learned_sirt_fallback_on_broken_network(ctx):
     torch.testing.assert_close(out, plain, rtol=1e-5, atol=1e-5)
 
 
+def test_solve_judges_by_residual_and_falls_back_to_plain(ctx):
+    """LearnedSIRT.solve: every image it returns within its budget is at or below the residual
+    target; with a network that spoils every stage and a budget too small to recover, each image
+    falls back to plain SIRT with the same budget (bitwise: the same kernels, batch-invariant)."""
+    from synkrotomo_amd.learned import Correction, LearnedSIRT, gpu_phantoms, residual
+    from synkrotomo_amd.projector import centred_bins, uniform_angles
+    N, A = 64, 45
+    R, rz = centred_bins(N)
+    geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
+    x = gpu_phantoms(ctx, np.random.default_rng(6), 3, N)
+    b = geo.fp(x)
+    net = Correction(ctx, seed=3)
+    with torch.no_grad():
+        net.w4.zero_()
+        net.b4.zero_()  # the identity correction: the stages are SIRT steps
+    ls = LearnedSIRT(geo, net, stages=3)
+    out, iters, fell = ls.solve(b, 0.05, 400, check_every=2)
+    res = residual(geo, out, b).cpu().numpy()
+    assert not bool(fell.any())
+    assert np.all(res <= 0.05), res
+    assert np.all((iters.cpu().numpy() >= 3) & (iters.cpu().numpy() <= 400))
+    with torch.no_grad():
+        # large random output weights: every stage adds high-frequency junk that 3 SIRT steps
+        # cannot remove (a constant offset would not do: one SIRT step removes it)
+        g = torch.Generator(device="cuda").manual_seed(9)
+        net.w4.copy_(torch.randn(net.w4.shape, device="cuda", generator=g) * 100.0)
+    out, iters, fell = ls.solve(b, 1e-4, 6)
+    plain = torch.zeros(3, N, N, device="cuda")
+    geo.sirt_(plain, b, 6)
+    learned, _ = ls.reconstruct(b, guard=False)
+    ls.tail_(learned, b, 3)
+    assert bool((iters == 6).all())
+    assert bool(fell.all()), (residual(geo, plain, b), residual(geo, learned, b))
+    torch.testing.assert_close(out, plain, rtol=0, atol=0)
+    with pytest.raises(ValueError):
+        ls.solve(b, 0.05, 2)
+
+
 def test_trained_network_gain_over_plain_sirt(ctx):
     """The shipped network against the method without it, on 16 held-out phantoms (512^2, 360
     angles, Poisson I0 = 1e4): after the 5 stages the image error is well below 5 plain SIRT
