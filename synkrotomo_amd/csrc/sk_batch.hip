@@ -129,7 +129,7 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fps_persist(const BatchArgs a, uns
 }
 
 template <bool EXACT, int NC, bool DR1, int PPT>
-__global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
+__global__ __launch_bounds__(256, NC > 0 ? 8 : 1) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
   __shared__ BpLineLds lsh[BP_LB];
   __shared__ unsigned item_sh;
@@ -145,8 +145,13 @@ __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned*
     const long per_img = (long)B.bp_tx * B.bp_ty;
     const int bz = (int)(local / per_img);
     const long rem = local - (long)bz * per_img;
-    const int ty = (int)(rem / B.bp_tx);
-    const int tx = (int)(rem - (long)ty * B.bp_tx);
+    int ty = (int)(rem / B.bp_tx);
+    int tx = (int)(rem - (long)ty * B.bp_tx);
+    if (B.bp.order) {  // the image's tiles dearest first, as in k_bp
+      const unsigned t = B.bp.order[rem];
+      tx = (int)(t & 0xffffu);
+      ty = (int)(t >> 16);
+    }
     bp_item<EXACT, BP_SIRT, W_REF, NC, DR1, PPT>(B.bp, u32(tx), u32(ty), u32(bz), win, lsh);
   }
 }
