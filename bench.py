@@ -326,7 +326,7 @@ def batched_leg(ctx, iters: int = 30) -> dict:
     return out
 
 
-def learned_leg(ctx, n_test: int = 32) -> dict:
+def learned_leg(ctx, n_test: int = 64) -> dict:
     """Time to a data residual, learned SIRT (5 x (SIRT step + BF16 CNN), trained weights in
     synkrotomo_amd/weights) vs plain SIRT, on held-out 512^2 phantoms, 360 angles, 725 bins,
     Poisson noise I0 = 1e4.  Targets: the literal 1e-2 and 1.1x each phantom's noise floor
