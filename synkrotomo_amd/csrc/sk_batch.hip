@@ -129,7 +129,7 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fps_persist(const BatchArgs a, uns
 }
 
 template <bool EXACT, int NC, bool DR1, int PPT>
-__global__ __launch_bounds__(256, NC > 0 ? 8 : 1) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
+__global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
   __shared__ BpLineLds lsh[BP_LB];
   __shared__ unsigned item_sh;
