@@ -250,15 +250,15 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
   G.n_steep = build_lines(angles_host, A, G.lines, N);
   {
     // Staged forward projector: the intercepts of 256 consecutive bins span 255*drho/|cost|
-    // columns, plus their rounding; a block of 16 rows shares one segment start, so the
-    // segment adds the drift of 15 rows (15*|slope|) and 5 columns (the pair and the floors).
+    // columns, plus their rounding; a block of FPS_ROWS rows shares one segment start, so the
+    // segment adds the drift of FPS_ROWS - 1 rows and 5 columns (the pair and the floors).
     double inv = 0.0, slope = 0.0;
     for (const Line& L : G.lines) {
       inv = std::max(inv, 1.0 / std::fabs((double)L.cost));
       slope = std::max(slope, std::fabs((double)L.slope));
     }
     const double span = 255.0 * drho * inv + 4.0 * (std::fabs((double)rho0) + (double)R * drho) * inv * 0x1p-23;
-    G.fps_w = (int)std::ceil(span + 15.0 * slope + 0.01) + 5;
+    G.fps_w = (int)std::ceil(span + (FPS_ROWS - 1) * slope + 0.01) + 5;
   }
   hipError_t e = hipMalloc(&G.d_lines, sizeof(Line) * (size_t)A);
   if (e == hipSuccess)

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "sk_common.h"
 
@@ -833,10 +834,17 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
 // rays can touch, at most FPS_MAXW) into LDS with coalesced dword loads, then every lane forms
 // its ray's row steps from LDS with the fast-mode step of fp_row_add.
 constexpr int FPS_RAYS = 256;
-constexpr int FPS_ROWS = 16;
+constexpr int FPS_ROWS = 12;
 constexpr int FPS_CPL = 6;                       // staged floats per lane and row
 constexpr int FPS_MAXW = FPS_CPL * 64;           // widest segment (floats)
 constexpr int FPS_CS = FPS_ROWS + 1;             // LDS stride of a staged column (floats)
+constexpr int FPS_PF = FPS_ROWS / 2;             // rows of LDS reads in flight ahead of the sums
+
+// f(integral_constant<int, I>) for I in the sequence, in order (compile-time unrolling).
+template <int... I, class F>
+__device__ __forceinline__ void unroll_seq(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
 
 typedef __attribute__((address_space(3))) float lds_float;
 
@@ -903,11 +911,11 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   };
   float acc = 0.0f;
   const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
-  // The LDS reads run eight rows ahead of the sums (rolling window of eight row registers).
-  float xbs[8], vas[8], vcs[8];
+  // The LDS reads run FPS_PF rows ahead of the sums (rolling window of FPS_PF row registers).
+  float xbs[FPS_PF], vas[FPS_PF], vcs[FPS_PF];
   auto fetch = [&](auto kc, float tab, int bbase) {
     constexpr int K = decltype(kc)::value;
-    constexpr int S = K & 7;
+    constexpr int S = K % FPS_PF;
     xbs[S] = bcast16<K>(tab) + icpt;
     // LDS byte address of the pair's first column in row slot 0
     const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * FPS_CS) + bbase);
@@ -920,34 +928,20 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
     acc = POS ? fp_row_add<true, NT>(acc, xb, xt, vas[S], vcs[S], rs)
               : fp_row_add<false, NT>(acc, xb, xt, vcs[S], vas[S], rs);
   };
-  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + 8
+  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + FPS_PF
     constexpr int K = decltype(kc)::value;
     sum_row(K);
-    fetch(std::integral_constant<int, K + 8>{}, tab, bbase);
+    fetch(std::integral_constant<int, K + FPS_PF>{}, tab, bbase);
     __builtin_amdgcn_sched_barrier(0);
   };
   auto compute = [&](int buf, float tab, int sb) {
     // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
     const int bbase = (buf * FPS_MAXW * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
-    fetch(std::integral_constant<int, 0>{}, tab, bbase);
-    fetch(std::integral_constant<int, 1>{}, tab, bbase);
-    fetch(std::integral_constant<int, 2>{}, tab, bbase);
-    fetch(std::integral_constant<int, 3>{}, tab, bbase);
-    fetch(std::integral_constant<int, 4>{}, tab, bbase);
-    fetch(std::integral_constant<int, 5>{}, tab, bbase);
-    fetch(std::integral_constant<int, 6>{}, tab, bbase);
-    fetch(std::integral_constant<int, 7>{}, tab, bbase);
+    unroll_seq(std::make_integer_sequence<int, FPS_PF>{}, [&](auto kc) { fetch(kc, tab, bbase); });
     __builtin_amdgcn_sched_barrier(0);
-    step(std::integral_constant<int, 0>{}, tab, bbase);
-    step(std::integral_constant<int, 1>{}, tab, bbase);
-    step(std::integral_constant<int, 2>{}, tab, bbase);
-    step(std::integral_constant<int, 3>{}, tab, bbase);
-    step(std::integral_constant<int, 4>{}, tab, bbase);
-    step(std::integral_constant<int, 5>{}, tab, bbase);
-    step(std::integral_constant<int, 6>{}, tab, bbase);
-    step(std::integral_constant<int, 7>{}, tab, bbase);
+    unroll_seq(std::make_integer_sequence<int, FPS_ROWS - FPS_PF>{}, [&](auto kc) { step(kc, tab, bbase); });
 #pragma unroll
-    for (int S = 0; S < 8; ++S) sum_row(S);
+    for (int S = 0; S < FPS_PF; ++S) sum_row(S);
   };
   float tab, tabn;
   int sb, sbn;
