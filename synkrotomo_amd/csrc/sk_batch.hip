@@ -107,7 +107,7 @@ __global__ __launch_bounds__(FP_BLOCK) void k_fp_persist(const BatchArgs a, unsi
 // item of FPS_RAYS bins of one line of one image, claimed one at a time from the global
 // counter (the claim is broadcast through LDS; its two barriers are workgroup-uniform), in
 // the same lines-fastest order.  NBUF = 1: FPS_CS * FPS_MAXW floats of dynamic LDS.
-__global__ __launch_bounds__(FPS_RAYS) void k_fps_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
+__global__ __launch_bounds__(FPS_RAYS, 8) void k_fps_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float4 seg4[];
   __shared__ unsigned item_sh;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0u;  // the bp queue of this iteration
