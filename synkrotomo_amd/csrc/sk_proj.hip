@@ -11,7 +11,7 @@
 //               transposed copy of the image, the reference's own frame trick
 //               (forwardprojection.fut:80-82).
 //   k_fp_staged A.x, staged (variants 5, 6): a workgroup = 256 consecutive bins of one
-//               line.  It walks the frame rows in blocks of 16: the four waves load the
+//               line.  It walks the frame rows in blocks of FPS_ROWS (12): the four waves load the
 //               block's image-row segment (the columns any of its rays can touch) with
 //               coalesced dword loads into LDS, then every lane forms its ray's row steps
 //               from LDS.  Same per-step expressions as k_fp (bitwise equal results).
