@@ -53,3 +53,11 @@ def test_single_angle_backprojection(ctx, theta):
 def test_step_tie_line(ctx):
     _check(ctx, 2048, np.array([TIE_ANGLE], np.float32))
     _check(ctx, 2048, np.array([0.9, TIE_ANGLE, 2.2], np.float32))
+
+
+@pytest.mark.parametrize("N,theta", [(1500, 0.35), (1500, 2.0), (4096, 0.6), (4096, 1.3)])
+def test_single_angle_other_sizes(ctx, N, theta):
+    """Image sizes whose tile edges fall off the powers of two (1500: h = 750) and the largest
+    sweep size (4096: binades up to [2048, 4096))."""
+    _check(ctx, N, np.array([theta], np.float32))
+
