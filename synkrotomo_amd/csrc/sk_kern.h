@@ -405,9 +405,10 @@ __host__ __device__ inline int bp_window_bins(int ppt, float drho, int ncand) {
 struct alignas(16) BpLineLds {
   int w0;
   float ci, si, cst;
-  float rs;      // 1/|slope| on LINE_NEWTON lines (fast path), else 0
   float dx;      // hoisted fast path: the exact |x_top - x_bot| of every candidate of the tile
                  // (tile_binade), else 0
+  float slope;   // the line's (read here with the rest of the record: no scalar load per line)
+  float rs;      // 1/|slope| on LINE_NEWTON lines (fast path), else 0
   float cost;    // the line's (window staging)
   float wscale;  // factor of the staged values: lbase, or lbase/dx when dx > 0
   int orig;
@@ -511,6 +512,7 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         e.cst = -(fmaf(L.cost, L.fc, L.sint * L.fs) + P.rho0) * idr;
       }
       e.cost = L.cost;
+      e.slope = L.slope;
       e.orig = L.orig;
       e.rs = (L.flags & LINE_NEWTON) ? 1.0f / fabsf(L.slope) : 0.0f;
       const int eb = ebin[fl ? 1 : 0];
@@ -577,8 +579,8 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         // lines; 5, 6: as 0, 1 in a tile with one binade (tile_binade), with the exact step.
         auto line_body = [&](auto kind, int jj) {
           constexpr int KIND = decltype(kind)::value;
-          const float slope = P.lines[l0 + jj].slope;
           const BpLineLds lc = lsh[jj];
+          const float slope = lc.slope;
           // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
           const float tb = part == 0 ? fmaf(lc.ci, xh[0], lc.cst) : fmaf(lc.si, yh[0], lc.cst);
           // All of the lane's pixels first compute their candidate index and read their
