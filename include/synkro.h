@@ -153,8 +153,10 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
 /* No reference counterpart: time the kernel variants on this geometry and keep the fastest
  * for it.  fp: 0/1/2 = a wave per 64/128/256 bins (1, 2 or 4 rays per lane), bin blocks
  * fastest in the grid; 3/4 = 1 or 2 rays per lane, lines fastest; 5/6 = 256 bins per
- * workgroup with each block of 16 image rows staged in LDS (double / single buffered; fast
- * mode, deltarho up to about 1, otherwise they run variant 3).  bp: 16x16, 16x32, 16x64 pixel
+ * workgroup with each block of 12 image rows staged in LDS (double / single buffered; fast
+ * mode, deltarho up to about 1, otherwise they run variant 3); 7 = 64 bins of four lines
+ * adjacent in slope per workgroup, staged the same way (fast mode, geometries whose line
+ * groups fit four chunks of 64 columns, otherwise variant 6).  bp: 16x16, 16x32, 16x64 pixel
  * tiles.  Variants compute identical results.  Synchronizes `stream`. */
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
 /* Apply previously tuned variants (-1 = the context default). */

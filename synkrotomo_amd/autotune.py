@@ -22,7 +22,7 @@ def cache_path() -> str:
 
 
 # Bumped whenever the meaning of the variant numbers changes, so old cache entries are ignored.
-VARIANT_TABLE = 3
+VARIANT_TABLE = 4
 
 
 def _key(geom: Geometry) -> str:

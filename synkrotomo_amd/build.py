@@ -3,7 +3,8 @@
     python -m synkrotomo_amd.build
 
 Each csrc/*.hip file is compiled to an object in parallel, then linked into
-synkrotomo_amd/lib/libsynkro.so.  Flags:
+synkrotomo_amd/lib/libsynkro.so; probes/*.hip (stand-alone measurement programs, not part of
+the library) become executables of the same name in synkrotomo_amd/lib/.  Flags:
   -ffp-contract=off       every multiply and add stays a separate IEEE operation (the
                           projector's pixel indices depend on it)
   -packed-fp32-ops off    no packed FP32 instructions are generated
@@ -21,6 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libsynkro.so")
+PROBES = os.path.join(PKG, "probes")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SYNKRO_ARCH", "gfx950")
 
@@ -40,12 +42,25 @@ def _compile(src: str) -> str:
     return obj
 
 
+def _probe(src: str) -> str:
+    exe = os.path.join(LIBDIR, os.path.splitext(os.path.basename(src))[0])
+    cmd = [HIPCC, *CFLAGS, src, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    workers = min(len(srcs), max(1, (os.cpu_count() or 4)))
+    probes = sorted(glob.glob(os.path.join(PROBES, "*.hip")))
+    workers = min(len(srcs) + len(probes), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        probe_jobs = [ex.submit(_probe, p) for p in probes]
         objs = list(ex.map(_compile, srcs))
+        for j in probe_jobs:
+            j.result()
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

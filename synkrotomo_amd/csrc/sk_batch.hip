@@ -396,7 +396,7 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
 
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
   if (!g) return SK_EINVAL;
-  if (fp_variant < -1 || fp_variant > 6 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
+  if (fp_variant < -1 || fp_variant > 7 || bp_variant < -1 || bp_variant > 2) return SK_EINVAL;
   g->g.fp_variant = fp_variant;
   g->g.bp_variant = bp_variant;
   return SK_OK;
@@ -436,7 +436,7 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
   };
   const bool verbose = getenv("SYNKRO_VERBOSE") != nullptr;
   float best = 1e30f, ms = 0.0f;
-  for (int v = 0; v <= 6 && !rc; v++) {
+  for (int v = 0; v <= 7 && !rc; v++) {
     rc = time_it(true, v, &ms);
     if (!rc && ms < best) { best = ms; best_fp = v; }
     if (!rc && verbose) fprintf(stderr, "sk_autotune: fp variant %d: %.3f ms / 3\n", v, ms);

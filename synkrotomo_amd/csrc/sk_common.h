@@ -45,6 +45,10 @@ struct Geom {
   bool bp_two_cand = false;
   // Row-segment width (floats) the staged forward projector needs for 256 consecutive bins.
   int fps_w = 0;
+  // Grouped staged forward projector: line groups (int4 entries, fpg_groups in sk_geom.hip),
+  // their count and the largest staged chunk count among them (null: not applicable).
+  int4* d_fpg = nullptr;
+  int n_fpg = 0, fpg_cpl = 0;
   std::vector<Line> lines;   // host copy, steep lines then flat lines
   Line* d_lines = nullptr;
   // backprojector intercepts (s*drho + rho0)/cost per table line j and bin s, [A][R] (null when

@@ -69,6 +69,63 @@ static float newton_min_slope(int N) {
 // frame columns lie in one binade (tile_binade: steep lines over the tile's 16 columns, flat
 // lines over its 64-row band), so the tiles of a launch differ in cost; starting the dear ones
 // first leaves cheap blocks for the end of the launch (longest-first list scheduling).
+// Line groups of the grouped staged forward projector: the lines of each frame and sign of
+// cost in slope order, in groups of FPG_LINES.  A group's staged width for FPG_BINS consecutive bins covers the spread
+// of its rays' x = i*slope + (rho0 + r*drho)/cost over the FPS_ROWS rows of any block: the
+// spread (a maximum minus a minimum of functions linear in the block's first row) is convex
+// in that row, so the blocks at the image's first and last rows bound it; plus the rounding
+// of the intercepts and of x_bot, the pair and the floors (as for fps_w).  Entry: x = first
+// line | CPL << 24 (chunks of 64 columns), y, z, w = the other lines or -1.  Empty when a
+// group would need more than FPG_MAXCPL chunks.
+static std::vector<int4> fpg_groups(const std::vector<Line>& lines, int n_steep, int N, int R, float rho0,
+                                    float drho, int* cpl_max) {
+  std::vector<int4> out;
+  *cpl_max = 0;
+  const int A = (int)lines.size(), h = N / 2, nb = (R + FPG_BINS - 1) / FPG_BINS;
+  for (int cls = 0; cls < 4; ++cls) {  // frame x sign of cost: lines of opposite directions
+    const int frame = cls >> 1;          // put bin r on opposite sides of the frame
+    const bool pos = cls & 1;
+    std::vector<int> idx;
+    for (int j = frame ? n_steep : 0; j < (frame ? A : n_steep); ++j)
+      if ((lines[j].cost > 0.0f) == pos) idx.push_back(j);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return lines[a].slope < lines[b].slope; });
+    for (size_t g0 = 0; g0 < idx.size(); g0 += FPG_LINES) {
+      const int n = (int)std::min<size_t>(FPG_LINES, idx.size() - g0);
+      double spread = 0.0, inv = 0.0;
+      for (int b = 0; b < nb; ++b) {
+        const double r0 = (double)b * FPG_BINS, r1 = r0 + FPG_BINS - 1;
+        for (const int ib : {-h, h - 1}) {
+          double lo = 1e300, hi = -1e300;
+          for (int k = 0; k < n; ++k) {
+            const Line& L = lines[idx[g0 + k]];
+            inv = std::max(inv, 1.0 / std::fabs((double)L.cost));
+            for (const int i : {ib, ib + FPS_ROWS - 1})
+              for (const double r : {r0, r1}) {
+                const double x = i * (double)L.slope + ((double)rho0 + r * drho) / (double)L.cost;
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+              }
+          }
+          spread = std::max(spread, hi - lo);
+        }
+      }
+      const double round = 4.0 * (std::fabs((double)rho0) + (double)R * drho) * inv * 0x1p-23 + (h + 16) * 0x1p-22;
+      const int w = (int)std::ceil(spread + round + 0.01) + 5;
+      const int cpl = std::max(2, (w + 63) / 64);
+      if (cpl > FPG_MAXCPL) {
+        out.clear();
+        *cpl_max = 0;
+        return out;
+      }
+      *cpl_max = std::max(*cpl_max, cpl);
+      int l[FPG_LINES];
+      for (int k = 0; k < FPG_LINES; ++k) l[k] = k < n ? idx[g0 + k] : -1;
+      out.push_back(make_int4(l[0] | (cpl << 24), l[1], l[2], l[3]));
+    }
+  }
+  return out;
+}
+
 static std::vector<unsigned> bp_tile_orders(int N, int n_steep, int A, size_t off[3]) {
   std::vector<unsigned> all;
   const int h = N / 2, ntx = (N + BP_TW - 1) / BP_TW;
@@ -278,10 +335,19 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
     if (e == hipSuccess)
       e = hipMemcpy(G.d_bp_order, ord.data(), sizeof(unsigned) * ord.size(), hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess && A < (1 << 24)) {
+    const std::vector<int4> grp = fpg_groups(G.lines, G.n_steep, N, R, rho0, drho, &G.fpg_cpl);
+    if (!grp.empty() && grp.size() < 65536) {
+      e = hipMalloc(&G.d_fpg, sizeof(int4) * grp.size());
+      if (e == hipSuccess) e = hipMemcpy(G.d_fpg, grp.data(), sizeof(int4) * grp.size(), hipMemcpyHostToDevice);
+      G.n_fpg = (int)grp.size();
+    }
+  }
   if (e != hipSuccess) {
     if (G.d_lines) (void)hipFree(G.d_lines);
     if (G.d_inter) (void)hipFree(G.d_inter);
     if (G.d_bp_order) (void)hipFree(G.d_bp_order);
+    if (G.d_fpg) (void)hipFree(G.d_fpg);
     delete g;
     return hip_fail(c, e, "geometry upload");
   }
@@ -298,6 +364,7 @@ void sk_geom_destroy(sk_geom* g) {
   if (g->g.d_lines) (void)hipFree(g->g.d_lines);
   if (g->g.d_inter) (void)hipFree(g->g.d_inter);
   if (g->g.d_bp_order) (void)hipFree(g->g.d_bp_order);
+  if (g->g.d_fpg) (void)hipFree(g->g.d_fpg);
   if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
   if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);
   delete g;

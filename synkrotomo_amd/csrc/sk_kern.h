@@ -156,6 +156,7 @@ struct FpParams {
   float* out;           // [batch][A][R]
   const float* b;       // FP_SIRT: measured sinogram
   const float* rowinv;  // FP_SIRT: [A][R]
+  const int4* groups;   // grouped staged variant: the geometry's line groups (else null)
 };
 
 // Lane k of every 16-lane row of v, broadcast to the whole wave when v holds the same 16
@@ -855,25 +856,26 @@ __device__ __forceinline__ int bcast16i(int v) {
   return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 
-template <bool POS, int NBUF, bool NT = false>
+// The staged walk of one ray per lane over the workgroup's rows i0 .. i1 (uniform) in blocks
+// of FPS_ROWS; the wave computes the blocks that meet its own rows i0w .. i1w.  CPL: staged
+// chunks of 64 columns per row.  seg_start(ib, tab) (uniform): the block's segment start,
+// one column left of floor(xbot) of the lowest ray of the workgroup over the block's rows.
+template <bool POS, int NBUF, bool NT, int CPL, class SegStart>
 __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
                                           int i0w, int i1w, float slope, float icpt,
-                                          float icpt_lo, float* seg, float rs = 0.0f) {
+                                          SegStart seg_start, float* seg, float rs = 0.0f) {
   const int tid = threadIdx.x;
   const int l16 = tid & 15;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int RPW = FPS_ROWS / (FPS_RAYS / 64);  // staged rows per wave
-  float st[RPW][FPS_CPL];
-  // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segments all start at
-  // image column sb, one left of floor(x) of the lowest ray over the block's rows (xbot is
-  // rounded from i*slope + icpt exactly as each lane rounds it, and rounding is monotone, so
-  // every lane's floor(xbot) - 1 lies at or right of it).
+  constexpr int W = 64 * CPL;                      // staged columns
+  float st[RPW][CPL];
+  // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segment starts at
+  // image column sb = seg_start(ib, tab).
   auto block_rows = [&](int ib, float& tab, int& sb) {
     tab = (float)(ib + l16) * slope;
-    const int sv = floor_to_int(tab + icpt_lo) + h - 1;
-    // one start for the whole block: the lower of its first and last rows' (monotone in i)
-    sb = min(__builtin_amdgcn_readlane(sv, 0), __builtin_amdgcn_readlane(sv, FPS_ROWS - 1));
+    sb = seg_start(ib, tab);
   };
   // Wave w stages rows RPW*w .. RPW*w + RPW-1 of the block, lane l columns s0 + l + 64m:
   // coalesced dword loads through the row's descriptor (zero outside the image row and
@@ -886,13 +888,13 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
       const int vo = (s0 + lane) << 2;
       if (s0 >= 0) {  // wave-uniform: non-negative offsets, the constant parts as immediates
 #pragma unroll
-        for (int m = 0; m < FPS_CPL; ++m)
+        for (int m = 0; m < CPL; ++m)
           st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, vo + 256 * m, 0, 0));
       } else {
         // segment starting left of the image: each load's full offset in its own register
         // (a negative one is out of range as an unsigned offset and reads 0)
 #pragma unroll
-        for (int m = 0; m < FPS_CPL; ++m) {
+        for (int m = 0; m < CPL; ++m) {
           int o = vo + 256 * m;
           asm volatile("" : "+v"(o));
           st[t][m] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rr, o, 0, 0));
@@ -906,9 +908,9 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   auto store = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < RPW; ++t) {
-      float* base = seg + (size_t)buf * FPS_MAXW * FPS_CS + lane * FPS_CS + RPW * wave + t;
+      float* base = seg + (size_t)buf * W * FPS_CS + lane * FPS_CS + RPW * wave + t;
 #pragma unroll
-      for (int m = 0; m < FPS_CPL; ++m) base[64 * FPS_CS * m] = st[t][m];
+      for (int m = 0; m < CPL; ++m) base[64 * FPS_CS * m] = st[t][m];
     }
   };
   float acc = 0.0f;
@@ -938,7 +940,7 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   };
   auto compute = [&](int buf, float tab, int sb) {
     // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
-    const int bbase = (buf * FPS_MAXW * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
+    const int bbase = (buf * W * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
     unroll_seq(std::make_integer_sequence<int, FPS_PF>{}, [&](auto kc) { fetch(kc, tab, bbase); });
     __builtin_amdgcn_sched_barrier(0);
     unroll_seq(std::make_integer_sequence<int, FPS_ROWS - FPS_PF>{}, [&](auto kc) { step(kc, tab, bbase); });
@@ -1009,20 +1011,117 @@ __device__ __forceinline__ void fps_item(const FpParams& P, int j, int blk, int 
     i1w = __builtin_amdgcn_readfirstlane(i1w);
   }
   const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+  // The block's segment start: xbot is rounded from i*slope + icpt exactly as each lane
+  // rounds it and rounding is monotone, so the lowest ray's floor(xbot) - 1 over the block's
+  // first and last rows lies at or left of every lane's.
+  auto seg_start = [&](int ib, float tab) {
+    const int sv = floor_to_int(tab + icpt_lo) + h - 1;
+    return min(__builtin_amdgcn_readlane(sv, 0), __builtin_amdgcn_readlane(sv, FPS_ROWS - 1));
+  };
   float acc = 0.0f;
   if (i0 < i1) {
     const bool nt = L.flags & LINE_NEWTON;
     const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
     if ((L.flags & LINE_SLOPE_POS) && nt)
-      acc = fps_walk<true, NBUF, true>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg, rs);
+      acc = fps_walk<true, NBUF, true, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else if (L.flags & LINE_SLOPE_POS)
-      acc = fps_walk<true, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+      acc = fps_walk<true, NBUF, false, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
     else if (nt)
-      acc = fps_walk<false, NBUF, true>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg, rs);
+      acc = fps_walk<false, NBUF, true, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else
-      acc = fps_walk<false, NBUF>(src, N, h, i0, i1, i0w, i1w, slope, icpt, icpt_lo, seg);
+      acc = fps_walk<false, NBUF, false, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
   }
   if (r >= P.R) return;  // after the walk: every lane took part in its barriers
+  const float s = acc * L.lbase;
+  const size_t o = (size_t)bz * P.A * P.R + (size_t)L.orig * P.R + r;
+  if constexpr (MODE == FP_PLAIN)
+    P.out[o] = s;
+  else
+    P.out[o] = P.rowinv[(size_t)L.orig * P.R + r] * (P.b[o] - s);  // SIRT.fut:33-36
+}
+
+// ------------------------------------------------------- grouped staged forward projection
+// A workgroup = FPG_BINS (64) consecutive bins of up to FPG_LINES (4) lines of one frame that
+// are adjacent in slope (the geometry's group table); wave w walks line w's 64 rays.  The
+// rays of such lines at the same bins cross a frame row within a few columns of each other,
+// so a block's staged segment (every column any of the 256 rays touches) is CPL = 2..4 chunks
+// of 64 columns, where 256 bins of one line need 6: the staging per ray-row drops to a third
+// or a half.  The walk and each ray's sums are fps_walk's (bitwise equal results).
+constexpr int FPG_LINES = FPS_RAYS / 64;
+constexpr int FPG_BINS = 64;
+constexpr int FPG_MAXCPL = 4;
+
+// A group table entry: the lines' table indices (x: first line | CPL << 24; y, z, w: the
+// others, -1 past the group's size).
+__device__ __forceinline__ int fpg_line(const int4& g, int k) {
+  return k == 0 ? (g.x & 0xffffff) : (k == 1 ? g.y : (k == 2 ? g.z : g.w));
+}
+
+template <int MODE, int NBUF, int CPL>
+__device__ __forceinline__ void fpg_item(const FpParams& P, const int4 grp, int blk, int bz, float* seg) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int nl = 1 + (grp.y >= 0) + (grp.z >= 0) + (grp.w >= 0);  // lines fill slots 0 .. nl-1
+  const bool active = wave < nl;
+  const int N = P.N, h = N >> 1;
+  const int first = blk * FPG_BINS, r = first + lane;
+  const int last = min(first + FPG_BINS - 1, P.R - 1);
+  const Line L = P.lines[fpg_line(grp, active ? wave : 0)];
+  const float cost = L.cost, slope = L.slope;
+  const float icpt = (P.rho0 + (float)r * P.drho) / cost;  // forwardprojection.fut:33-34
+  // Lane k < 2*nl describes line k/2 at its first (k even) or last ray: the workgroup's row
+  // hull (valid rays) and the block segment starts (every lane's ray, valid or not: the
+  // intercepts are monotone in the bin, so the extremes are the block's first and last bin).
+  // Lanes past 2*nl repeat lane 0.
+  const int kk = lane < 2 * nl ? lane : 0;
+  const Line Lk = P.lines[fpg_line(grp, kk >> 1)];
+  const float ia = (P.rho0 + (float)first * P.drho) / Lk.cost;
+  const float iz = (P.rho0 + (float)(first + FPG_BINS - 1) * P.drho) / Lk.cost;
+  const float sl_k = Lk.slope, ilo_k = fminf(ia, iz);
+  const int ro_k = (kk & 1) ? FPS_ROWS - 1 : 0;
+  int i0 = 0, i1 = 0, i0w = 0, i1w = 0;
+  {
+    int c0, c1;
+    row_clip(sl_k, (kk & 1) ? (P.rho0 + (float)last * P.drho) / Lk.cost : ia, h, c0, c1);
+    i0 = __builtin_amdgcn_readlane(c0, 0);
+    i1 = __builtin_amdgcn_readlane(c1, 0);
+#pragma unroll
+    for (int k = 1; k < 2 * FPG_LINES; ++k) {
+      i0 = min(i0, __builtin_amdgcn_readlane(c0, k));
+      i1 = max(i1, __builtin_amdgcn_readlane(c1, k));
+    }
+    // the wave's own rows: its line's lanes (2*wave, 2*wave + 1)
+    if (active) {
+      i0w = min(__builtin_amdgcn_readlane(c0, 2 * wave), __builtin_amdgcn_readlane(c0, 2 * wave + 1));
+      i1w = max(__builtin_amdgcn_readlane(c1, 2 * wave), __builtin_amdgcn_readlane(c1, 2 * wave + 1));
+    } else {
+      i0w = i1w = i0;  // no line: stages its rows, computes nothing
+    }
+  }
+  // Segment start of the block at row ib: the lowest floor(xbot) - 1 over the group's lines
+  // at the block's first and last rows, xbot rounded exactly as the lanes round it.
+  auto seg_start = [&](int ib, float) {
+    const int sv = floor_to_int((float)(ib + ro_k) * sl_k + ilo_k) + h - 1;
+    int sb = __builtin_amdgcn_readlane(sv, 0);
+#pragma unroll
+    for (int k = 1; k < 2 * FPG_LINES; ++k) sb = min(sb, __builtin_amdgcn_readlane(sv, k));
+    return sb;
+  };
+  const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+  float acc = 0.0f;
+  if (i0 < i1) {
+    const bool nt = L.flags & LINE_NEWTON;
+    const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
+    if ((L.flags & LINE_SLOPE_POS) && nt)
+      acc = fps_walk<true, NBUF, true, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+    else if (L.flags & LINE_SLOPE_POS)
+      acc = fps_walk<true, NBUF, false, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+    else if (nt)
+      acc = fps_walk<false, NBUF, true, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+    else
+      acc = fps_walk<false, NBUF, false, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+  }
+  if (!active || r >= P.R) return;  // after the walk: every lane took part in its barriers
   const float s = acc * L.lbase;
   const size_t o = (size_t)bz * P.A * P.R + (size_t)L.orig * P.R + r;
   if constexpr (MODE == FP_PLAIN)

@@ -15,6 +15,9 @@
 //               block's image-row segment (the columns any of its rays can touch) with
 //               coalesced dword loads into LDS, then every lane forms its ray's row steps
 //               from LDS.  Same per-step expressions as k_fp (bitwise equal results).
+//   k_fp_grouped A.x, grouped staged (variant 7): a workgroup = 64 consecutive bins of four
+//               lines adjacent in slope, one wave per line, so the staged segment of a
+//               block is two to four chunks of 64 columns instead of six.  Same walk.
 //   k_bp  B.r   (futhark/backprojection.fut:20-93).  Pixel-parallel gather, no atomics: a
 //               block owns a 16 x 16*PPT pixel tile and walks the lines in order, staging
 //               for each block of 32 lines the tile's rho-window of the sinogram in LDS
@@ -81,6 +84,21 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
   fps_item<MODE, NBUF>(P, blockIdx.x, blockIdx.y, blockIdx.z, (float*)seg4);
 }
 
+// Grouped staged A.x (variant 7): grid (line groups, bin blocks of FPG_BINS, images).
+// Dynamic LDS: NBUF * FPS_CS * 64 * (the geometry's largest CPL) floats.
+template <int MODE, int NBUF>
+__global__ __launch_bounds__(FPS_RAYS, 8) void k_fp_grouped(FpParams P) {
+  extern __shared__ float4 seg4[];
+  const int4 grp = P.groups[blockIdx.x];
+  const int cpl = grp.x >> 24;  // uniform
+  if (cpl <= 2)
+    fpg_item<MODE, NBUF, 2>(P, grp, blockIdx.y, blockIdx.z, (float*)seg4);
+  else if (cpl == 3)
+    fpg_item<MODE, NBUF, 3>(P, grp, blockIdx.y, blockIdx.z, (float*)seg4);
+  else
+    fpg_item<MODE, NBUF, 4>(P, grp, blockIdx.y, blockIdx.z, (float*)seg4);
+}
+
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
 template <bool EXACT, int MODE, int WEIGHTS, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256, NC > 0 ? 8 : 1) void k_bp(BpParams P) {
@@ -110,7 +128,7 @@ int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipS
 
 FpParams fp_params(const Geom* g, const FpArgs& a) {
   return FpParams{g->d_lines, g->A, g->R, g->N, g->ncand, g->rho0, g->drho,
-                  a.img, a.imgT, a.out, a.b, a.rowinv};
+                  a.img, a.imgT, a.out, a.b, a.rowinv, g->d_fpg};
 }
 
 // fp variants: 0/1/2 = 1/2/4 rays per lane, bin blocks fastest; 3/4 = 1/2 rays per lane,
@@ -153,9 +171,29 @@ static void fp_go_staged(const Geom* g, const FpArgs& a, hipStream_t s) {
                      fp_params(g, a));
 }
 
+template <int MODE, int NBUF>
+static void fp_go_grouped(const Geom* g, const FpArgs& a, hipStream_t s) {
+  const unsigned nb = (unsigned)((g->R + FPG_BINS - 1) / FPG_BINS);
+  const size_t lds = sizeof(float) * NBUF * FPS_CS * 64 * (size_t)g->fpg_cpl;
+  hipLaunchKernelGGL((k_fp_grouped<MODE, NBUF>), dim3(g->n_fpg, nb, a.batch), dim3(FPS_RAYS), lds, s,
+                     fp_params(g, a));
+}
+
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s) {
   const bool ex = c->exact != 0;
   int v = g->fp_variant >= 0 ? g->fp_variant : c->fp_variant;
+  if (v == 7) {
+    if (!ex && g->d_fpg && mode != FP_ONES_INV) {
+      // two buffers (one barrier per block) while they fit 8 workgroups per CU
+      const bool two = g->fpg_cpl <= 3;
+      if (mode == FP_PLAIN)
+        two ? fp_go_grouped<FP_PLAIN, 2>(g, a, s) : fp_go_grouped<FP_PLAIN, 1>(g, a, s);
+      else
+        two ? fp_go_grouped<FP_SIRT, 2>(g, a, s) : fp_go_grouped<FP_SIRT, 1>(g, a, s);
+      return launched(c, "forward projection kernel");
+    }
+    v = 6;  // no group table for this geometry: the one-line staged kernel
+  }
   if (v == 5 || v == 6) {
     if (!ex && g->fps_w > 0 && g->fps_w <= FPS_MAXW && mode != FP_ONES_INV) {
       if (mode == FP_PLAIN)

@@ -24,7 +24,7 @@ def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
     y = torch.rand(2, A, R, device="cuda")
     geo.set_variants(0, 0)
     f0, b0 = geo.fp(x), geo.bp(y)
-    for fv in range(7):
+    for fv in range(8):
         for bv in (0, 1, 2):
             geo.set_variants(fv, bv)
             torch.testing.assert_close(geo.fp(x), f0, rtol=0, atol=0)
@@ -35,6 +35,24 @@ def test_all_variants_bitwise_identical(ctx, N, A, R, rz, dr):
         assert rel_l2(f0[0].cpu().numpy(), ref) <= 1e-5
 
 
+@pytest.mark.parametrize("N,A,R,rz,dr", [(512, 360, 725, -362.0, 1.0), (102, 37, 300, -180.5, 0.9)])
+def test_sirt_every_fp_variant_bitwise(ctx, N, A, R, rz, dr):
+    """SIRT iterations (the residual form of A.x, fused) give the same bits with every forward
+    kernel shape; the angles in a shuffled order (the grouped kernel sorts its line groups)."""
+    ang = _uniform(A)[np.random.default_rng(5).permutation(A)]
+    geo = ctx.geometry(ang, rz, dr, R, N)
+    sino = torch.rand(A, R, device="cuda")
+    res = {}
+    for fv in (0, 6, 7):
+        geo.set_variants(fv, 2)
+        img = torch.zeros(N, N, device="cuda")
+        geo.sirt_(img, sino, 3)
+        res[fv] = img
+    geo.set_variants(-1, -1)
+    for fv in (6, 7):
+        torch.testing.assert_close(res[fv], res[0], rtol=0, atol=0)
+
+
 @pytest.mark.parametrize("N,A,R,rz,dr", [(64, 90, 91, -45.0, 1.0), (40, 23, 30, -29.0, 2.0)])
 def test_exact_mode_every_variant_bitwise_oracle(exact, N, A, R, rz, dr):
     """Exact mode is bitwise the oracle with every kernel shape and grid order."""
@@ -43,7 +61,7 @@ def test_exact_mode_every_variant_bitwise_oracle(exact, N, A, R, rz, dr):
     y = rng.random(A * R).astype(np.float32)
     fref, bref = oracle.fp(_uniform(A), rz, dr, R, x), oracle.bp(_uniform(A), rz, dr, N, y)
     geo = exact.geometry(_uniform(A), rz, dr, R, N)
-    for v in range(7):
+    for v in range(8):
         geo.set_variants(v, min(v, 2))
         np.testing.assert_array_equal(geo.fp(torch.from_numpy(x).cuda()).cpu().numpy().ravel(), fref)
         np.testing.assert_array_equal(geo.bp(torch.from_numpy(y).cuda()).cpu().numpy().ravel(), bref)
@@ -55,7 +73,7 @@ def test_autotune_cache(ctx, tmp_path, monkeypatch):
     monkeypatch.setenv("SYNKRO_TUNE_CACHE", str(tmp_path / "tune.json"))
     geo = ctx.geometry(_uniform(180), -181.0, 1.0, 363, 256)
     fv, bv = autotune.tuned(geo)
-    assert fv in range(7) and bv in (0, 1, 2)
+    assert fv in range(8) and bv in (0, 1, 2)
     assert (tmp_path / "tune.json").exists()
     assert autotune.tuned(geo) == (fv, bv)  # served from the cache
     geo.set_variants(-1, -1)
