@@ -71,7 +71,7 @@ static float newton_min_slope(int N) {
 // first leaves cheap blocks for the end of the launch (longest-first list scheduling).
 // Line groups of the grouped staged forward projector: the lines of each frame and sign of
 // cost in slope order, in groups of FPG_LINES.  A group's staged width for FPG_BINS consecutive bins covers the spread
-// of its rays' x = i*slope + (rho0 + r*drho)/cost over the FPS_ROWS rows of any block: the
+// of its rays' x = i*slope + (rho0 + r*drho)/cost over the FPG_ROWS rows of any block: the
 // spread (a maximum minus a minimum of functions linear in the block's first row) is convex
 // in that row, so the blocks at the image's first and last rows bound it; plus the rounding
 // of the intercepts and of x_bot, the pair and the floors (as for fps_w).  Entry: x = first
@@ -99,7 +99,7 @@ static std::vector<int4> fpg_groups(const std::vector<Line>& lines, int n_steep,
           for (int k = 0; k < n; ++k) {
             const Line& L = lines[idx[g0 + k]];
             inv = std::max(inv, 1.0 / std::fabs((double)L.cost));
-            for (const int i : {ib, ib + FPS_ROWS - 1})
+            for (const int i : {ib, ib + FPG_ROWS - 1})
               for (const double r : {r0, r1}) {
                 const double x = i * (double)L.slope + ((double)rho0 + r * drho) / (double)L.cost;
                 lo = std::min(lo, x);

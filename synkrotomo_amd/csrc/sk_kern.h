@@ -857,10 +857,10 @@ __device__ __forceinline__ int bcast16i(int v) {
 }
 
 // The staged walk of one ray per lane over the workgroup's rows i0 .. i1 (uniform) in blocks
-// of FPS_ROWS; the wave computes the blocks that meet its own rows i0w .. i1w.  CPL: staged
+// of ROWS (<= 16, a multiple of 4); the wave computes the blocks that meet its own rows i0w .. i1w.  CPL: staged
 // chunks of 64 columns per row.  seg_start(ib, tab) (uniform): the block's segment start,
 // one column left of floor(xbot) of the lowest ray of the workgroup over the block's rows.
-template <bool POS, int NBUF, bool NT, int CPL, class SegStart>
+template <bool POS, int NBUF, bool NT, int CPL, int ROWS, class SegStart>
 __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
                                           int i0w, int i1w, float slope, float icpt,
                                           SegStart seg_start, float* seg, float rs = 0.0f) {
@@ -868,7 +868,9 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   const int l16 = tid & 15;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  constexpr int RPW = FPS_ROWS / (FPS_RAYS / 64);  // staged rows per wave
+  constexpr int RPW = ROWS / (FPS_RAYS / 64);  // staged rows per wave
+  constexpr int CS = ROWS + 1;                 // LDS stride of a staged column (floats)
+  constexpr int PF = ROWS / 2;                 // rows of LDS reads in flight ahead of the sums
   constexpr int W = 64 * CPL;                      // staged columns
   float st[RPW][CPL];
   // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segment starts at
@@ -903,49 +905,49 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
     }
   };
   // LDS layout of a staged block: column-interleaved, column j of row slot r at j*CS + r
-  // (CS = FPS_ROWS + 1 spreads consecutive columns over the banks), so a lane's pair in row K
+  // (CS = ROWS + 1 spreads consecutive columns over the banks), so a lane's pair in row K
   // is one address plus the constant offsets K and K + CS.
   auto store = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < RPW; ++t) {
-      float* base = seg + (size_t)buf * W * FPS_CS + lane * FPS_CS + RPW * wave + t;
+      float* base = seg + (size_t)buf * W * CS + lane * CS + RPW * wave + t;
 #pragma unroll
-      for (int m = 0; m < CPL; ++m) base[64 * FPS_CS * m] = st[t][m];
+      for (int m = 0; m < CPL; ++m) base[64 * CS * m] = st[t][m];
     }
   };
   float acc = 0.0f;
   const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
-  // The LDS reads run FPS_PF rows ahead of the sums (rolling window of FPS_PF row registers).
-  float xbs[FPS_PF], vas[FPS_PF], vcs[FPS_PF];
+  // The LDS reads run PF rows ahead of the sums (rolling window of PF row registers).
+  float xbs[PF], vas[PF], vcs[PF];
   auto fetch = [&](auto kc, float tab, int bbase) {
     constexpr int K = decltype(kc)::value;
-    constexpr int S = K % FPS_PF;
+    constexpr int S = K % PF;
     xbs[S] = bcast16<K>(tab) + icpt;
     // LDS byte address of the pair's first column in row slot 0
-    const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * FPS_CS) + bbase);
+    const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * CS) + bbase);
     const lds_float* p = (const lds_float*)(uintptr_t)off;
     vas[S] = p[K];
-    vcs[S] = p[K + FPS_CS];
+    vcs[S] = p[K + CS];
   };
   auto sum_row = [&](int S) {
     const float xb = xbs[S], xt = xb + slope;
     acc = POS ? fp_row_add<true, NT>(acc, xb, xt, vas[S], vcs[S], rs)
               : fp_row_add<false, NT>(acc, xb, xt, vcs[S], vas[S], rs);
   };
-  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + FPS_PF
+  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + PF
     constexpr int K = decltype(kc)::value;
     sum_row(K);
-    fetch(std::integral_constant<int, K + FPS_PF>{}, tab, bbase);
+    fetch(std::integral_constant<int, K + PF>{}, tab, bbase);
     __builtin_amdgcn_sched_barrier(0);
   };
   auto compute = [&](int buf, float tab, int sb) {
     // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
-    const int bbase = (buf * W * FPS_CS + (h - sb - (POS ? 0 : 1)) * FPS_CS) * 4 + seg_lds;
-    unroll_seq(std::make_integer_sequence<int, FPS_PF>{}, [&](auto kc) { fetch(kc, tab, bbase); });
+    const int bbase = (buf * W * CS + (h - sb - (POS ? 0 : 1)) * CS) * 4 + seg_lds;
+    unroll_seq(std::make_integer_sequence<int, PF>{}, [&](auto kc) { fetch(kc, tab, bbase); });
     __builtin_amdgcn_sched_barrier(0);
-    unroll_seq(std::make_integer_sequence<int, FPS_ROWS - FPS_PF>{}, [&](auto kc) { step(kc, tab, bbase); });
+    unroll_seq(std::make_integer_sequence<int, ROWS - PF>{}, [&](auto kc) { step(kc, tab, bbase); });
 #pragma unroll
-    for (int S = 0; S < FPS_PF; ++S) sum_row(S);
+    for (int S = 0; S < PF; ++S) sum_row(S);
   };
   float tab, tabn;
   int sb, sbn;
@@ -954,15 +956,15 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   store(0);
   __syncthreads();
   int b = 0;
-  for (int ib = i0; ib < i1; ib += FPS_ROWS, ++b) {  // i0, i1 uniform: every lane reaches each barrier
-    const bool more = ib + FPS_ROWS < i1;
+  for (int ib = i0; ib < i1; ib += ROWS, ++b) {  // i0, i1 uniform: every lane reaches each barrier
+    const bool more = ib + ROWS < i1;
     if (more) {
-      block_rows(ib + FPS_ROWS, tabn, sbn);
-      load(ib + FPS_ROWS, sbn);
+      block_rows(ib + ROWS, tabn, sbn);
+      load(ib + ROWS, sbn);
     }
     __builtin_amdgcn_sched_barrier(0);  // the next block's loads stay ahead of the compute
     // A wave skips blocks outside its own rays' row range: there they add exactly 0.
-    if (ib + FPS_ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sb);
+    if (ib + ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sb);
     if constexpr (NBUF == 1) __syncthreads();
     if (more) store(NBUF == 2 ? ((b + 1) & 1) : 0);
     __syncthreads();
@@ -1023,13 +1025,13 @@ __device__ __forceinline__ void fps_item(const FpParams& P, int j, int blk, int 
     const bool nt = L.flags & LINE_NEWTON;
     const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
     if ((L.flags & LINE_SLOPE_POS) && nt)
-      acc = fps_walk<true, NBUF, true, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<true, NBUF, true, FPS_CPL, FPS_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else if (L.flags & LINE_SLOPE_POS)
-      acc = fps_walk<true, NBUF, false, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<true, NBUF, false, FPS_CPL, FPS_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
     else if (nt)
-      acc = fps_walk<false, NBUF, true, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<false, NBUF, true, FPS_CPL, FPS_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else
-      acc = fps_walk<false, NBUF, false, FPS_CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<false, NBUF, false, FPS_CPL, FPS_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
   }
   if (r >= P.R) return;  // after the walk: every lane took part in its barriers
   const float s = acc * L.lbase;
@@ -1046,10 +1048,12 @@ __device__ __forceinline__ void fps_item(const FpParams& P, int j, int blk, int 
 // rays of such lines at the same bins cross a frame row within a few columns of each other,
 // so a block's staged segment (every column any of the 256 rays touches) is CPL = 2..4 chunks
 // of 64 columns, where 256 bins of one line need 6: the staging per ray-row drops to a third
-// or a half.  The walk and each ray's sums are fps_walk's (bitwise equal results).
+// or a half, and blocks of FPG_ROWS = 16 rows fit (fewer per-block steps per row).  The walk
+// and each ray's sums are fps_walk's (bitwise equal results).
 constexpr int FPG_LINES = FPS_RAYS / 64;
 constexpr int FPG_BINS = 64;
 constexpr int FPG_MAXCPL = 4;
+constexpr int FPG_ROWS = 16;  // rows per staged block (8.5 KB of LDS per buffer at CPL 2)
 
 // A group table entry: the lines' table indices (x: first line | CPL << 24; y, z, w: the
 // others, -1 past the group's size).
@@ -1078,7 +1082,7 @@ __device__ __forceinline__ void fpg_item(const FpParams& P, const int4 grp, int 
   const float ia = (P.rho0 + (float)first * P.drho) / Lk.cost;
   const float iz = (P.rho0 + (float)(first + FPG_BINS - 1) * P.drho) / Lk.cost;
   const float sl_k = Lk.slope, ilo_k = fminf(ia, iz);
-  const int ro_k = (kk & 1) ? FPS_ROWS - 1 : 0;
+  const int ro_k = (kk & 1) ? FPG_ROWS - 1 : 0;
   int i0 = 0, i1 = 0, i0w = 0, i1w = 0;
   {
     int c0, c1;
@@ -1113,13 +1117,13 @@ __device__ __forceinline__ void fpg_item(const FpParams& P, const int4 grp, int 
     const bool nt = L.flags & LINE_NEWTON;
     const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
     if ((L.flags & LINE_SLOPE_POS) && nt)
-      acc = fps_walk<true, NBUF, true, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<true, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else if (L.flags & LINE_SLOPE_POS)
-      acc = fps_walk<true, NBUF, false, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<true, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
     else if (nt)
-      acc = fps_walk<false, NBUF, true, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<false, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
     else
-      acc = fps_walk<false, NBUF, false, CPL>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<false, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
   }
   if (!active || r >= P.R) return;  // after the walk: every lane took part in its barriers
   const float s = acc * L.lbase;

@@ -17,7 +17,8 @@
 //               from LDS.  Same per-step expressions as k_fp (bitwise equal results).
 //   k_fp_grouped A.x, grouped staged (variant 7): a workgroup = 64 consecutive bins of four
 //               lines adjacent in slope, one wave per line, so the staged segment of a
-//               block is two to four chunks of 64 columns instead of six.  Same walk.
+//               block is two to four chunks of 64 columns instead of six; blocks of 16
+//               rows.  Same walk.
 //   k_bp  B.r   (futhark/backprojection.fut:20-93).  Pixel-parallel gather, no atomics: a
 //               block owns a 16 x 16*PPT pixel tile and walks the lines in order, staging
 //               for each block of 32 lines the tile's rho-window of the sinogram in LDS
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
 }
 
 // Grouped staged A.x (variant 7): grid (line groups, bin blocks of FPG_BINS, images).
-// Dynamic LDS: NBUF * FPS_CS * 64 * (the geometry's largest CPL) floats.
+// Dynamic LDS: NBUF * (FPG_ROWS + 1) * 64 * (the geometry's largest CPL) floats.
 template <int MODE, int NBUF>
 __global__ __launch_bounds__(FPS_RAYS, 8) void k_fp_grouped(FpParams P) {
   extern __shared__ float4 seg4[];
@@ -174,7 +175,7 @@ static void fp_go_staged(const Geom* g, const FpArgs& a, hipStream_t s) {
 template <int MODE, int NBUF>
 static void fp_go_grouped(const Geom* g, const FpArgs& a, hipStream_t s) {
   const unsigned nb = (unsigned)((g->R + FPG_BINS - 1) / FPG_BINS);
-  const size_t lds = sizeof(float) * NBUF * FPS_CS * 64 * (size_t)g->fpg_cpl;
+  const size_t lds = sizeof(float) * NBUF * (FPG_ROWS + 1) * 64 * (size_t)g->fpg_cpl;
   hipLaunchKernelGGL((k_fp_grouped<MODE, NBUF>), dim3(g->n_fpg, nb, a.batch), dim3(FPS_RAYS), lds, s,
                      fp_params(g, a));
 }
@@ -184,12 +185,11 @@ int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s
   int v = g->fp_variant >= 0 ? g->fp_variant : c->fp_variant;
   if (v == 7) {
     if (!ex && g->d_fpg && mode != FP_ONES_INV) {
-      // two buffers (one barrier per block) while they fit 8 workgroups per CU
-      const bool two = g->fpg_cpl <= 3;
+      // two buffers, one barrier per block (17 KB of LDS at CPL 2: 8 workgroups per CU)
       if (mode == FP_PLAIN)
-        two ? fp_go_grouped<FP_PLAIN, 2>(g, a, s) : fp_go_grouped<FP_PLAIN, 1>(g, a, s);
+        fp_go_grouped<FP_PLAIN, 2>(g, a, s);
       else
-        two ? fp_go_grouped<FP_SIRT, 2>(g, a, s) : fp_go_grouped<FP_SIRT, 1>(g, a, s);
+        fp_go_grouped<FP_SIRT, 2>(g, a, s);
       return launched(c, "forward projection kernel");
     }
     v = 6;  // no group table for this geometry: the one-line staged kernel
