@@ -4,9 +4,10 @@
 // Batched SIRT (no reference counterpart; generalizes SIRT3D.fut:18-28, which maps SIRT over
 // the slices of one geometry): a problem set holds groups of independent problems, each
 // group with its own geometry (image size, angles, bins).  One iteration is two persistent
-// launches sized to fill the chip once: in k_fps_persist (fast mode) every workgroup pulls
-// (problem, image, line, 256-bin block) items of the staged forward projector from a global
-// work counter (k_fp_persist: waves pull 64-bin items of the per-wave walk, exact mode or
+// launches sized to fill the chip once: in k_fpg_persist (fast mode) every workgroup pulls
+// (problem, image, line group, 64-bin block) items of the grouped staged forward projector
+// from a global work counter (k_fps_persist: (problem, image, line, 256-bin block) items of
+// the one-line staged kernel) (k_fp_persist: waves pull 64-bin items of the per-wave walk, exact mode or
 // segments wider than the staging buffer); in k_bp_persist every block pulls (problem,
 // image, tile) items, queued largest-problem first so the tail of the queue is made of the
 // shortest items.  The fp kernel resets the bp counter and the
@@ -35,6 +36,7 @@ struct BatchProb {
   FpParams fp;
   BpParams bp;
   int batch;
+  int fp_units;       // forward items per bin group: lines (line groups for the grouped kernel)
   int fp_wb;          // wave-sized bin groups per line
   int bp_tx, bp_ty;   // tiles across / down
   long fp_begin, bp_begin;
@@ -128,6 +130,38 @@ __global__ __launch_bounds__(FPS_RAYS, 8) void k_fps_persist(const BatchArgs a, 
   }
 }
 
+// Grouped staged forward items (fast mode, every group has a line-group table): one
+// workgroup per item of FPG_BINS bins of one line group of one image, groups fastest, claimed
+// as in k_fps_persist.  Two buffers of (FPG_ROWS + 1) * 64 * CPL floats of dynamic LDS for the
+// largest CPL of the set.
+__global__ __launch_bounds__(FPS_RAYS, 8) void k_fpg_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
+  extern __shared__ float4 seg4[];
+  __shared__ unsigned item_sh;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0u;  // the bp queue of this iteration
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(&ctr[2], 1u);
+    __syncthreads();
+    const long item = item_sh;
+    __syncthreads();
+    if (item >= a.fp_total) break;
+    const BatchProb& B = a.p[find_prob<true>(a, item)];
+    const long local = item - B.fp_begin;
+    const long per_img = (long)B.fp_units * B.fp_wb;
+    const int bz = (int)(local / per_img);
+    const long rem = local - (long)bz * per_img;
+    const int wb = (int)(rem / B.fp_units);
+    const int g = (int)(rem - (long)wb * B.fp_units);
+    const int4 grp = B.fp.groups[u32(g)];
+    const int cpl = u32(grp.x >> 24);
+    if (cpl <= 2)
+      fpg_item<FP_SIRT, 2, 2>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+    else if (cpl == 3)
+      fpg_item<FP_SIRT, 2, 3>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+    else
+      fpg_item<FP_SIRT, 2, 4>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+  }
+}
+
 template <bool EXACT, int NC, bool DR1, int PPT>
 __global__ __launch_bounds__(256) void k_bp_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float2 win[];
@@ -162,13 +196,17 @@ struct BatchPlan {
   // of those on the batched configuration (3.82 s vs 4.02 s for mode 3 and 5.10 s for mode 0:
   // static striding leaves waves with the costliest items idle-waiting at the end).
   // SYNKRO_BATCH_FP_MODE overrides it; 4 = the staged kernel, the default in fast mode when
-  // every group's segments fit.
+  // every group's segments fit; 5 = the grouped staged kernel, the default in fast mode when
+  // every group has a line-group table.
   int fp_mode = 2;
+  int fpg_cpl = 0;  // mode 5: the largest chunk count of the set's line groups
   int bp_ppt = 4;
   unsigned* d_ctr = nullptr;
   int Wmax = 0;
   bool dr1 = true;
 };
+
+static size_t fpg_lds(int cpl) { return sizeof(float) * 2 * (FPG_ROWS + 1) * 64 * (size_t)cpl; }
 
 // Pixels per lane of the persistent backprojector (16 x 16*PPT tiles): 4 by default (16x64
 // tiles: 3.11 s vs 3.23 s with 16x32 on the four large groups of the batched configuration,
@@ -176,7 +214,10 @@ struct BatchPlan {
 
 template <bool EXACT, int NC, bool DR1>
 static int launch_iteration(Ctx* c, const BatchPlan& plan, int fp_grid, int bp_grid, hipStream_t s) {
-  if (plan.fp_mode == 4)
+  if (plan.fp_mode == 5)
+    hipLaunchKernelGGL(k_fpg_persist, dim3(fp_grid), dim3(FPS_RAYS), fpg_lds(plan.fpg_cpl), s, plan.args,
+                       plan.d_ctr);
+  else if (plan.fp_mode == 4)
     hipLaunchKernelGGL(k_fps_persist, dim3(fp_grid), dim3(FPS_RAYS), sizeof(float) * FPS_CS * FPS_MAXW, s,
                        plan.args, plan.d_ctr);
   else if (plan.fp_mode == 0)
@@ -252,19 +293,26 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
   BatchArgs& args = plan.args;
   args.n = count;
   const bool ex = c->exact != 0;
-  bool staged_fits = !ex;
+  bool staged_fits = !ex, grouped_fits = !ex;
   for (int i = 0; i < count; i++) {
     const Geom& G = problems[i].geom->g;
     staged_fits = staged_fits && G.fps_w > 0 && G.fps_w <= FPS_MAXW;
+    grouped_fits = grouped_fits && G.d_fpg != nullptr;
+    plan.fpg_cpl = std::max(plan.fpg_cpl, G.fpg_cpl);
   }
   if (staged_fits) plan.fp_mode = 4;
+  if (grouped_fits) plan.fp_mode = 5;
   if (const char* m = getenv("SYNKRO_BATCH_FP_MODE")) {
     const int want = atoi(m);
-    plan.fp_mode = (want == 4 && !staged_fits) ? plan.fp_mode : (want == 4 ? 4 : (want & 3));
+    if (want == 5 || want == 4)
+      plan.fp_mode = (want == 5 ? grouped_fits : staged_fits) ? want : plan.fp_mode;
+    else
+      plan.fp_mode = want & 3;
   }
   if (const char* m = getenv("SYNKRO_BATCH_BP_PPT")) plan.bp_ppt = atoi(m) == 2 ? 2 : 4;
   const int ppt = plan.bp_ppt;
-  const int fp_item_bins = plan.fp_mode == 4 ? FPS_RAYS : 64 * ((plan.fp_mode & 1) ? 2 : 1);
+  const int fp_item_bins =
+      plan.fp_mode == 5 ? FPG_BINS : (plan.fp_mode == 4 ? FPS_RAYS : 64 * ((plan.fp_mode & 1) ? 2 : 1));
   for (int oi = 0; oi < count; oi++) {
     const sk_problem& P = problems[order[oi]];
     const Geom& G = P.geom->g;
@@ -276,12 +324,13 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
     B.fp = fp_params(&G, FpArgs{P.d_img, imgT, q, P.d_sino, G.d_rowinv, P.batch});
     B.bp = bp_params(&G, BpArgs{q, P.d_img, imgT, G.d_colinv, P.batch}, ppt);
     B.batch = P.batch;
+    B.fp_units = plan.fp_mode == 5 ? G.n_fpg : G.A;
     B.fp_wb = (G.R + fp_item_bins - 1) / fp_item_bins;
     B.bp_tx = (G.N + BP_TW - 1) / BP_TW;
     B.bp_ty = (G.N + 16 * ppt - 1) / (16 * ppt);
     B.fp_begin = args.fp_total;
     B.bp_begin = args.bp_total;
-    args.fp_total += (long)P.batch * G.A * B.fp_wb;
+    args.fp_total += (long)P.batch * B.fp_units * B.fp_wb;
     args.bp_total += (long)P.batch * B.bp_tx * B.bp_ty;
     plan.Wmax = std::max(plan.Wmax, B.bp.W);
     plan.dr1 = plan.dr1 && G.bp_two_cand;
@@ -299,8 +348,9 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
                            (const void*)k_fp_persist<true, 1, true>, (const void*)k_fp_persist<true, 2, true>};
   const void* kfp_fast[4] = {(const void*)k_fp_persist<false, 1, false>, (const void*)k_fp_persist<false, 2, false>,
                              (const void*)k_fp_persist<false, 1, true>, (const void*)k_fp_persist<false, 2, true>};
-  const void* kfp = plan.fp_mode == 4 ? (const void*)k_fps_persist
-                                      : (ex ? kfp_ex[plan.fp_mode & 3] : kfp_fast[plan.fp_mode & 3]);
+  const void* kfp = plan.fp_mode == 5   ? (const void*)k_fpg_persist
+                    : plan.fp_mode == 4 ? (const void*)k_fps_persist
+                                        : (ex ? kfp_ex[plan.fp_mode & 3] : kfp_fast[plan.fp_mode & 3]);
   const void* kbp2 =
       plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, 2> : (const void*)k_bp_persist<false, 2, true, 2>)
                : (ex ? (const void*)k_bp_persist<true, 0, false, 2> : (const void*)k_bp_persist<false, 0, false, 2>);
@@ -308,8 +358,9 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
       plan.dr1 ? (ex ? (const void*)k_bp_persist<true, 2, true, 4> : (const void*)k_bp_persist<false, 2, true, 4>)
                : (ex ? (const void*)k_bp_persist<true, 0, false, 4> : (const void*)k_bp_persist<false, 0, false, 4>);
   const void* kbp = plan.bp_ppt == 4 ? kbp4 : kbp2;
-  rc = plan.fp_mode == 4 ? persistent_grid(c, kfp, FPS_RAYS, sizeof(float) * FPS_CS * FPS_MAXW, &fp_grid)
-                         : persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
+  rc = plan.fp_mode == 5   ? persistent_grid(c, kfp, FPS_RAYS, fpg_lds(plan.fpg_cpl), &fp_grid)
+       : plan.fp_mode == 4 ? persistent_grid(c, kfp, FPS_RAYS, sizeof(float) * FPS_CS * FPS_MAXW, &fp_grid)
+                           : persistent_grid(c, kfp, FP_BLOCK, 0, &fp_grid);
   if (!rc) rc = persistent_grid(c, kbp, 256, lds, &bp_grid);
   if (rc) return rc;
   if (getenv("SYNKRO_VERBOSE"))

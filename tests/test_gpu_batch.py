@@ -79,15 +79,19 @@ def test_autotune_cache(ctx, tmp_path, monkeypatch):
     geo.set_variants(-1, -1)
 
 
+@pytest.mark.parametrize("fp_mode", ["", "4", "2"], ids=["grouped", "staged", "per-wave"])
 @pytest.mark.parametrize("variants", [(-1, -1), (2, 2)])
 @pytest.mark.parametrize("specs", [
     [(32, 30, 46, -22.5, 3), (64, 45, 91, -45.0, 2), (48, 20, 68, -33.5, 4)],
     [(256, 180, 363, -181.0, 2), (512, 360, 725, -362.0, 1)],
 ], ids=["small", "256-512"])
-def test_sirt_batched_equals_per_problem(ctx, specs, variants):
+def test_sirt_batched_equals_per_problem(ctx, specs, variants, fp_mode, monkeypatch):
     """Uneven groups in one persistent launch sequence == each problem run on its own, with
-    the per-problem runs on the default and on other kernel shapes than the batched ones."""
+    the per-problem runs on the default and on other kernel shapes than the batched ones, and
+    the batched forward items of each kind (grouped staged by default)."""
     from synkrotomo_amd.projector import sirt_batched
+    if fp_mode:
+        monkeypatch.setenv("SYNKRO_BATCH_FP_MODE", fp_mode)
     gen = torch.Generator(device="cuda").manual_seed(7)
     groups, singles = [], []
     for N, A, R, rz, b in specs:
