@@ -208,7 +208,7 @@ def stream3d_leg(ctx, S: int = 512, N: int = 512, A: int = 512, iters: int = 10,
     from synkrotomo_amd.stream3d import sirt3d_streamed
     R, rz = centred_bins(N)
     geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
-    tuned(geo)
+    tuned(geo, batch=chunk)  # the calls run chunk (and S) slices at a time
     proj = np.random.default_rng(5).random((S, A, R), dtype=np.float32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -265,7 +265,7 @@ def normalize_leg(ctx, A: int = 2048, R: int = 2896, batch: int = 16, reps: int 
 def _batched_case(ctx, specs, iters: int, seed: int) -> dict:
     """specs: [(N, angles, images)], centred unit bins.  Batched SIRT (one persistent launch
     sequence, one hipGraph) against the same problems group by group through sk_sirt with
-    each geometry's autotuned kernel shapes; each path runs twice and the second (warm) run
+    each geometry's kernel shapes autotuned on its batch; each path runs twice and the second (warm) run
     is timed.  Sinograms are uniform random (timing is data-independent)."""
     import torch
     from synkrotomo_amd.projector import centred_bins, sirt_batched, uniform_angles
@@ -275,7 +275,7 @@ def _batched_case(ctx, specs, iters: int, seed: int) -> dict:
     for n, a, cnt in specs:
         r, rz = centred_bins(n)
         geo = ctx.geometry(uniform_angles(a), rz, 1.0, r, n)
-        geo.autotune()
+        geo.autotune(min(cnt, 64))  # tuned on the group's own batch of images
         groups.append((geo, torch.zeros(cnt, n, n, device="cuda"),
                        torch.rand(cnt, a, r, device="cuda", generator=gen)))
         updates += 2.0 * iters * cnt * n * n * a

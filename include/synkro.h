@@ -159,6 +159,10 @@ int sk_sirt_batched(sk_ctx* ctx, const sk_problem* problems, int count, int iter
  * groups fit four chunks of 64 columns, otherwise variant 6).  bp: 16x16, 16x32, 16x64 pixel
  * tiles.  Variants compute identical results.  Synchronizes `stream`. */
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream);
+/* No reference counterpart: sk_autotune timed on calls of `batch` images (1..1024) -- the
+ * batch the caller will run, since batches of small images favour other shapes than one
+ * image (sk_autotune is batch 1).  Keeps the fastest pair for the geometry. */
+int sk_autotune_batch(sk_ctx* ctx, sk_geom* g, int batch, int* fp_variant, int* bp_variant, void* stream);
 /* Apply previously tuned variants (-1 = the context default). */
 int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant);
 

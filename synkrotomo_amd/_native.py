@@ -48,6 +48,7 @@ SIGNATURES = {
     "sk_sirt_step": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sk_sirt_batched": (_i, [_vp, _vp, _i, _i, _vp]),
     "sk_autotune": (_i, [_vp, _vp, _ip, _ip, _vp]),
+    "sk_autotune_batch": (_i, [_vp, _vp, _i, _ip, _ip, _vp]),
     "sk_geom_set_variants": (_i, [_vp, _i, _i]),
     "sk_conv32": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sk_conv_in": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),

@@ -1,8 +1,9 @@
 """Kernel-variant autotuning with a persistent cache.
 
-``tuned(geom)`` applies the cached choice for the geometry's shape (image size, angle
-count, bins, bin width, device architecture) or runs ``sk_autotune`` on the GPU and stores
-the result.  The cache is a JSON file at $SYNKRO_TUNE_CACHE, default
+``tuned(geom, batch)`` applies the cached choice for the geometry's shape (image size, angle
+count, bins, bin width, device architecture) and the calls' batch size (rounded up to a power
+of two, at most 64) or runs ``sk_autotune_batch`` on the GPU and stores the result: batches of
+small images fill the chip where one image does not, and favour other kernel shapes.  The cache is a JSON file at $SYNKRO_TUNE_CACHE, default
 ~/.cache/synkrotomo_amd/tune.json.  Variants only change tile shapes and rays per lane --
 every variant computes identical results -- so a stale cache can only cost time.
 """
@@ -22,12 +23,20 @@ def cache_path() -> str:
 
 
 # Bumped whenever the meaning of the variant numbers changes, so old cache entries are ignored.
-VARIANT_TABLE = 4
+VARIANT_TABLE = 5
 
 
-def _key(geom: Geometry) -> str:
+def batch_bucket(batch: int) -> int:
+    """The batch size variants are tuned at for calls of `batch` images."""
+    b = 1
+    while b < min(int(batch), 64):
+        b *= 2
+    return b
+
+
+def _key(geom: Geometry, batch: int = 1) -> str:
     arch = torch.cuda.get_device_properties(geom.ctx.device).gcnArchName.split(":")[0]
-    return f"v{VARIANT_TABLE}/{arch}/N{geom.N}/A{geom.A}/R{geom.R}/d{geom.drho:g}"
+    return f"v{VARIANT_TABLE}/{arch}/N{geom.N}/A{geom.A}/R{geom.R}/d{geom.drho:g}/b{batch_bucket(batch)}"
 
 
 def _load() -> dict:
@@ -50,10 +59,11 @@ def _store(table: dict) -> None:
         pass  # read-only home: tuning still applies to this process
 
 
-def tuned(geom: Geometry, retune: bool = False) -> tuple[int, int]:
-    """Apply (and if needed measure) the fastest kernel variants for this geometry."""
+def tuned(geom: Geometry, retune: bool = False, batch: int = 1) -> tuple[int, int]:
+    """Apply (and if needed measure) the fastest kernel variants for this geometry on calls of
+    `batch` images."""
     table = _load()
-    key = _key(geom)
+    key = _key(geom, batch)
     if not retune and key in table:
         fv, bv = table[key]
         try:
@@ -61,7 +71,7 @@ def tuned(geom: Geometry, retune: bool = False) -> tuple[int, int]:
             return fv, bv
         except ValueError:
             pass  # an entry the library no longer accepts: measure again
-    fv, bv = geom.autotune()
+    fv, bv = geom.autotune(batch_bucket(batch))
     table[key] = [fv, bv]
     _store(table)
     return fv, bv

@@ -454,13 +454,20 @@ int sk_geom_set_variants(sk_geom* g, int fp_variant, int bp_variant) {
 }
 
 int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void* stream) {
+  return sk_autotune_batch(ctx, g, 1, fp_variant, bp_variant, stream);
+}
+
+int sk_autotune_batch(sk_ctx* ctx, sk_geom* g, int batch, int* fp_variant, int* bp_variant, void* stream) {
   if (!ctx) return SK_EINVAL;
   Ctx* c = &ctx->c;
   if (!g) return fail(c, SK_EINVAL, "geometry is NULL");
+  if (batch < 1 || batch > 1024) return fail(c, SK_EINVAL, "autotune batch must be 1..1024 (got %d)", batch);
   SK_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   Geom& G = g->g;
-  const size_t nimg = (size_t)G.N * G.N, nsino = (size_t)G.A * G.R;
+  // batch images and sinograms: the variants are timed on the calls' own batch size (small
+  // images in a batch fill the chip where one does not, and favour other shapes)
+  const size_t nimg = (size_t)batch * G.N * G.N, nsino = (size_t)batch * G.A * G.R;
   // own buffers (the projector calls below use the context scratch for x^T)
   float *img = nullptr, *sino = nullptr;
   SK_HIP(c, hipMalloc(&img, sizeof(float) * (nimg + nsino)));
@@ -477,7 +484,7 @@ int sk_autotune(sk_ctx* ctx, sk_geom* g, int* fp_variant, int* bp_variant, void*
     const int reps = 3;
     for (int k = 0; k <= reps; k++) {
       if (k == 1) SK_HIP(c, hipEventRecord(e0, s));
-      int r = fp ? sk_fp(ctx, g, 1, img, sino, stream) : sk_bp(ctx, g, 1, sino, img, stream);
+      int r = fp ? sk_fp(ctx, g, batch, img, sino, stream) : sk_bp(ctx, g, batch, sino, img, stream);
       if (r) return r;
     }
     SK_HIP(c, hipEventRecord(e1, s));

@@ -233,11 +233,13 @@ class Geometry:
                                     int(iterations), _stream(self.ctx.device)), self.ctx.handle, "SIRT")
         return img
 
-    def autotune(self):
-        """Time the kernel variants on this geometry, keep the fastest; returns (fp, bp)."""
+    def autotune(self, batch: int = 1):
+        """Time the kernel variants on this geometry on calls of `batch` images, keep the
+        fastest; returns (fp, bp)."""
         fv, bv = ctypes.c_int(), ctypes.c_int()
-        nat.check(nat.lib().sk_autotune(self.ctx.handle, self.handle, ctypes.byref(fv), ctypes.byref(bv),
-                                        _stream(self.ctx.device)), self.ctx.handle, "autotune")
+        nat.check(nat.lib().sk_autotune_batch(self.ctx.handle, self.handle, int(batch), ctypes.byref(fv),
+                                              ctypes.byref(bv), _stream(self.ctx.device)),
+                  self.ctx.handle, "autotune")
         return fv.value, bv.value
 
     def set_variants(self, fp_variant: int = -1, bp_variant: int = -1) -> None:

@@ -1,4 +1,6 @@
 """Kernel variants, the autotuner and batched persistent SIRT."""
+import ctypes
+import json
 import numpy as np
 import pytest
 import torch
@@ -76,7 +78,23 @@ def test_autotune_cache(ctx, tmp_path, monkeypatch):
     assert fv in range(8) and bv in (0, 1, 2)
     assert (tmp_path / "tune.json").exists()
     assert autotune.tuned(geo) == (fv, bv)  # served from the cache
+    # batches are tuned and cached apart: batch 20 is timed on 32 images
+    fb, bb = autotune.tuned(geo, batch=20)
+    assert fb in range(8) and bb in (0, 1, 2)
+    keys = json.load(open(tmp_path / "tune.json"))
+    assert sorted(k.rsplit("/", 1)[1] for k in keys) == ["b1", "b32"]
+    assert autotune.tuned(geo, batch=17) == (fb, bb)
+    assert [autotune.batch_bucket(b) for b in (1, 2, 3, 16, 17, 64, 500)] == [1, 2, 4, 16, 32, 64, 64]
     geo.set_variants(-1, -1)
+
+
+def test_autotune_batch_rejects_bad_batch(ctx):
+    from synkrotomo_amd import _native as nat
+    geo = ctx.geometry(_uniform(12), -11.5, 1.0, 24, 16)
+    fv, bv = ctypes.c_int(), ctypes.c_int()
+    for bad in (0, -3, 1025):
+        rc = nat.lib().sk_autotune_batch(ctx.handle, geo.handle, bad, ctypes.byref(fv), ctypes.byref(bv), None)
+        assert rc != 0
 
 
 @pytest.mark.parametrize("fp_mode", ["", "4", "2"], ids=["grouped", "staged", "per-wave"])
