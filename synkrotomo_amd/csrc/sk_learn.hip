@@ -230,15 +230,23 @@ __global__ __launch_bounds__(256) void k_conv_out_mfma(int H, int W, const __bf1
   bf16x8 wb;
   for (int i = 0; i < 8; ++i) wb[i] = (__bf16)0.0f;
   if (m < 9) wb = *(const bf16x8*)(w + m * NCH + 8 * kq);
-  for (int g = wv; g < CO_SLOTS / 16; g += 4) {
-    const int slot = g * 16 + m;  // A row m: halo pixel `slot`
+  // wave wv takes pixel groups wv, wv + 4, ...: all of its loads are issued before the MFMAs
+  constexpr int NG = CO_SLOTS / 16, GPW = (NG + 3) / 4;
+  bf16x8 a[GPW];
+#pragma unroll
+  for (int k = 0; k < GPW; ++k) {
+    const int slot = (wv + 4 * k) * 16 + m;  // A row m: halo pixel `slot`
     const int hy = slot / (CT + 2), hx = slot - hy * (CT + 2);
     const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
-    bf16x8 a;
-    for (int i = 0; i < 8; ++i) a[i] = (__bf16)0.0f;
+    for (int i = 0; i < 8; ++i) a[k][i] = (__bf16)0.0f;
     if (slot < (CT + 2) * (CT + 2) && gy >= 0 && gy < H && gx >= 0 && gx < W)
-      a = *(const bf16x8*)(h + (img + (size_t)gy * W + gx) * NCH + 8 * kq);
-    const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+      a[k] = *(const bf16x8*)(h + (img + (size_t)gy * W + gx) * NCH + 8 * kq);
+  }
+#pragma unroll
+  for (int k = 0; k < GPW; ++k) {
+    const int g = wv + 4 * k;
+    if (g >= NG) break;  // wave-uniform
+    const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k], wb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
     // C: lane holds U[pixel g*16 + 4kq + i][tap m]
     if (m < 9)
       for (int i = 0; i < 4; ++i) U[(g * 16 + 4 * kq + i) * 9 + m] = c[i];

@@ -46,13 +46,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "learned_sirt_512_360.npz")
 
 
-def geometry(ctx):
-    """The 512^2 / 360-angle / 725-bin geometry with its autotuned kernel shapes (plain and
-    learned SIRT run the same projector kernels)."""
+def geometry(ctx, batch: int = 16):
+    """The 512^2 / 360-angle / 725-bin geometry with its kernel shapes autotuned on batches of
+    `batch` images (plain and learned SIRT run the same projector kernels on the same batches)."""
     from .autotune import tuned
     R, rz = centred_bins(N)
     geo = ctx.geometry(uniform_angles(A), rz, 1.0, R, N)
-    tuned(geo)
+    tuned(geo, batch=batch)
     return geo
 
 
@@ -65,7 +65,7 @@ def attenuation_scale(ctx, geo, rng) -> float:
 
 def train(args) -> int:
     ctx = Context.default(0)
-    geo = geometry(ctx)
+    geo = geometry(ctx, args.batch)
     rng = np.random.default_rng(TRAIN_SEED)
     scale = attenuation_scale(ctx, geo, np.random.default_rng(TRAIN_SEED + 1))
     gen = torch.Generator(device="cuda").manual_seed(NOISE_SEED)
@@ -204,7 +204,7 @@ def evaluate(ctx, weights: str, n_test: int, batch: int = 16, max_iters: int = 3
     image errors are recorded after every iteration up to `dense` and every `stride` iterations
     beyond, up to `max_iters` or until every image is at or below 1e-2: an iteration count past
     `dense` is an upper bound within `stride`."""
-    geo = geometry(ctx)
+    geo = geometry(ctx, batch)
     net = Correction(ctx)
     meta = net.load(weights)
     scale = float(meta["scale"])
