@@ -62,7 +62,7 @@ def test_operator_calls_reject_bad_arguments(ctx):
     assert L.sk_bp(h, gh, 70000, p(sino), p(img), None) == 1
     assert L.sk_sirt(h, gh, 1, p(img), p(sino), -1, None) == 1
     assert "iterations" in L.sk_last_error(h).decode()
-    assert L.sk_geom_set_variants(gh, 7, 0) == 1
+    assert L.sk_geom_set_variants(gh, 8, 0) == 1
     assert L.sk_set_option(h, 99, 0) == 1
     assert L.sk_sparse_build(h, gh, 7, ctypes.byref(ctypes.c_void_p()), None) == 1
     assert L.sk_compare(h, 0, p(img), p(img), ctypes.c_float(0.05), None, None, None, None) == 1
