@@ -19,8 +19,10 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 
 import numpy as np
+
 import torch
 
 from . import _native as nat
@@ -33,6 +35,21 @@ BF16 = torch.bfloat16
 
 def _chk(rc, ctx, what):
     nat.check(rc, ctx.handle, what)
+
+
+_BF16_CACHE: dict = {}
+
+
+def _bf16_of(w: torch.Tensor) -> torch.Tensor:
+    """w in BF16, contiguous; cached per tensor (a weak reference: a new tensor at a recycled
+    id is never mistaken for it) until its data or its version counter changes."""
+    key = id(w)
+    hit = _BF16_CACHE.get(key)
+    stamp = (w.data_ptr(), w._version, tuple(w.shape))
+    if hit is None or hit[0]() is not w or hit[1] != stamp:
+        hit = (weakref.ref(w), stamp, w.detach().to(BF16).contiguous())
+        _BF16_CACHE[key] = hit
+    return hit[2]
 
 
 class _Correction(torch.autograd.Function):
@@ -48,10 +65,9 @@ class _Correction(torch.autograd.Function):
         cin = 1 if inp.dim() == 3 else inp.shape[-1]
         dev = res.device
         s = _stream(ctx.device)
-        w1b = w1.to(BF16).contiguous()          # [32, 3, 3] or [32, 3, 3, 2]
-        w2b = w2.to(BF16).contiguous()          # [32, 3, 3, 32]
-        w3b = w3.to(BF16).contiguous()
-        w4b = w4.to(BF16).contiguous()          # [3, 3, 32]
+        # BF16 copies of the weights: w1 [32, 3, 3] or [32, 3, 3, 2], w2 and w3 [32, 3, 3, 32],
+        # w4 [3, 3, 32]; converted again only when a weight has changed
+        w1b, w2b, w3b, w4b = (_bf16_of(w) for w in (w1, w2, w3, w4))
         h1 = torch.empty((B, H, W, NCH), dtype=BF16, device=dev)
         h2 = torch.empty_like(h1)
         h3 = torch.empty_like(h1)
@@ -305,7 +321,9 @@ class LearnedSIRT:
                     # stage's (the kernels are deterministic: the same bits)
                     xp = xs.clone()
             xc = self.net(xs, xs - x) if self.net.cin == 2 else self.net(xs)
-            bad = ~torch.isfinite(xc).reshape(B, -1).all(dim=1)
+            # a non-finite pixel makes its image's sum non-finite (a float sum reduction, cheaper
+            # than reducing a per-pixel test); a finite image whose sum overflows is dropped too
+            bad = ~torch.isfinite(xc.reshape(B, -1).sum(dim=1))
             x = torch.where(bad.view(B, 1, 1), xs, xc)
         if not guard:
             return x, torch.zeros(B, dtype=torch.bool, device=b.device)
