@@ -132,7 +132,8 @@ __global__ __launch_bounds__(FPS_RAYS, 8) void k_fps_persist(const BatchArgs a, 
 
 // Grouped staged forward items (fast mode, every group has a line-group table): one
 // workgroup per item of FPG_BINS bins of one line group of one image, groups fastest, claimed
-// as in k_fps_persist.  Two buffers of (FPG_ROWS + 1) * 64 * CPL floats of dynamic LDS for the
+// as in k_fps_persist; the per-row step only (the exact per-block step's registers would
+// spill here).  Two buffers of (FPG_ROWS + 1) * 64 * CPL floats of dynamic LDS for the
 // largest CPL of the set.
 __global__ __launch_bounds__(FPS_RAYS, 8) void k_fpg_persist(const BatchArgs a, unsigned* __restrict__ ctr) {
   extern __shared__ float4 seg4[];
@@ -154,11 +155,11 @@ __global__ __launch_bounds__(FPS_RAYS, 8) void k_fpg_persist(const BatchArgs a, 
     const int4 grp = B.fp.groups[u32(g)];
     const int cpl = u32(grp.x >> 24);
     if (cpl <= 2)
-      fpg_item<FP_SIRT, 2, 2>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+      fpg_item<FP_SIRT, 2, 2, false>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
     else if (cpl == 3)
-      fpg_item<FP_SIRT, 2, 3>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+      fpg_item<FP_SIRT, 2, 3, false>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
     else
-      fpg_item<FP_SIRT, 2, 4>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
+      fpg_item<FP_SIRT, 2, 4, false>(B.fp, grp, u32(wb), u32(bz), (float*)seg4);
   }
 }
 

@@ -49,6 +49,8 @@ struct Geom {
   // their count and the largest staged chunk count among them (null: not applicable).
   int4* d_fpg = nullptr;
   int n_fpg = 0, fpg_cpl = 0;
+  // and its exact per-block step per line and binade (k_fp_step_table in sk_geom.hip)
+  float2* d_fp_steps = nullptr;
   std::vector<Line> lines;   // host copy, steep lines then flat lines
   Line* d_lines = nullptr;
   // backprojector intercepts (s*drho + rho0)/cost per table line j and bin s, [A][R] (null when

@@ -193,6 +193,30 @@ static int build_lines(const float* ang, int A, std::vector<Line>& out, int N = 
   return (int)steep.size();
 }
 
+// The forward projector's exact per-block step per line j and binade e (fps_walk): (D, 1/D)
+// with D = binade_step(slope, e) -- 0 where |slope| is a rounding tie or below half a unit at
+// e -- and 1/D by the expression fp_row_add evaluates for a row whose |x_top - x_bot| is D
+// (the Newton step from 1/|slope| on LINE_NEWTON lines, else the hardware reciprocal), so a
+// block that takes the exact step forms every fraction bit for bit as the per-row step does.
+__global__ __launch_bounds__(256) void k_fp_step_table(const Line* __restrict__ lines, int A,
+                                                       float2* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= A * FP_STEP_BINADES) return;
+  const int j = i / FP_STEP_BINADES, e = i - j * FP_STEP_BINADES;
+  const Line L = lines[j];
+  const float D = binade_step(L.slope, e);
+  float r = 0.0f;
+  if (D > 0.0f) {
+    if (L.flags & LINE_NEWTON) {
+      const float rs = 1.0f / fabsf(L.slope);
+      r = fmaf(rs, fmaf(-D, rs, 1.0f), rs);
+    } else {
+      r = fast_rcp(D);
+    }
+  }
+  out[i] = make_float2(D, r);
+}
+
 // inter[j][s] = (s*drho + rho0) / cost_j (backprojection.fut:57-58), IEEE division: the values
 // the backprojector stages for its candidate windows.
 __global__ __launch_bounds__(256) void k_inter_table(const Line* __restrict__ lines, int R, float rho0,
@@ -341,6 +365,13 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
       e = hipMalloc(&G.d_fpg, sizeof(int4) * grp.size());
       if (e == hipSuccess) e = hipMemcpy(G.d_fpg, grp.data(), sizeof(int4) * grp.size(), hipMemcpyHostToDevice);
       G.n_fpg = (int)grp.size();
+      if (e == hipSuccess) e = hipMalloc(&G.d_fp_steps, sizeof(float2) * (size_t)A * FP_STEP_BINADES);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fp_step_table, dim3((A * FP_STEP_BINADES + 255) / 256), dim3(256), 0, nullptr,
+                           G.d_lines, A, G.d_fp_steps);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     }
   }
   if (e != hipSuccess) {
@@ -348,6 +379,7 @@ int sk_geom_create(sk_ctx* ctx, const float* angles_host, int A, float rho0, flo
     if (G.d_inter) (void)hipFree(G.d_inter);
     if (G.d_bp_order) (void)hipFree(G.d_bp_order);
     if (G.d_fpg) (void)hipFree(G.d_fpg);
+    if (G.d_fp_steps) (void)hipFree(G.d_fp_steps);
     delete g;
     return hip_fail(c, e, "geometry upload");
   }
@@ -365,6 +397,7 @@ void sk_geom_destroy(sk_geom* g) {
   if (g->g.d_inter) (void)hipFree(g->g.d_inter);
   if (g->g.d_bp_order) (void)hipFree(g->g.d_bp_order);
   if (g->g.d_fpg) (void)hipFree(g->g.d_fpg);
+  if (g->g.d_fp_steps) (void)hipFree(g->g.d_fp_steps);
   if (g->g.d_rowinv) (void)hipFree(g->g.d_rowinv);
   if (g->g.d_colinv) (void)hipFree(g->g.d_colinv);
   delete g;

@@ -157,7 +157,11 @@ struct FpParams {
   const float* b;       // FP_SIRT: measured sinogram
   const float* rowinv;  // FP_SIRT: [A][R]
   const int4* groups;   // grouped staged variant: the geometry's line groups (else null)
+  const float2* steps;  // grouped staged variant: per line, (D_e, 1/D_e) for binades e (else null)
 };
+
+// Binades e = 0 .. FP_STEP_BINADES-1 of the forward projector's exact per-block step table.
+constexpr int FP_STEP_BINADES = 16;
 
 // Lane k of every 16-lane row of v, broadcast to the whole wave when v holds the same 16
 // values in each row (DPP row_newbcast; the compiler folds it into the consuming add).
@@ -857,13 +861,15 @@ __device__ __forceinline__ int bcast16i(int v) {
 }
 
 // The staged walk of one ray per lane over the workgroup's rows i0 .. i1 (uniform) in blocks
-// of ROWS (<= 16, a multiple of 4); the wave computes the blocks that meet its own rows i0w .. i1w.  CPL: staged
-// chunks of 64 columns per row.  seg_start(ib, tab) (uniform): the block's segment start,
-// one column left of floor(xbot) of the lowest ray of the workgroup over the block's rows.
+// of ROWS (<= 16, a multiple of 4); the wave computes the blocks that meet its own rows i0w ..
+// i1w.  CPL: staged chunks of 64 columns per row.  seg_start(ib, tab) (uniform): the block's
+// segment start, one column left of floor(xbot) of the lowest ray of the workgroup over the
+// block's rows.  steps: the line's exact step per binade (fp_step_table), or null.
 template <bool POS, int NBUF, bool NT, int CPL, int ROWS, class SegStart>
 __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0, int i1,
                                           int i0w, int i1w, float slope, float icpt,
-                                          SegStart seg_start, float* seg, float rs = 0.0f) {
+                                          SegStart seg_start, float* seg, float rs = 0.0f,
+                                          const float2* __restrict__ steps = nullptr) {
   const int tid = threadIdx.x;
   const int l16 = tid & 15;
   const int lane = tid & 63;
@@ -917,37 +923,77 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   };
   float acc = 0.0f;
   const int seg_lds = (int)(uintptr_t)(lds_float*)seg;  // LDS address of the block
+  // Exact per-block step.  When every x_bot and x_top of the wave's rays over the block's rows
+  // lies in one binade [2^e, 2^(e+1)), e >= 0 (tile_binade of the block's column range), all
+  // of them are multiples of u = 2^(e-23), so x_top = x_bot + RN_u(slope) and the row's
+  // |x_top - x_bot| is the line's constant D_e (binade_step; no rounding tie, D_e > 0).  The
+  // row then needs no x_top, difference or reciprocal: with fr = fract(x_bot) (exact for
+  // |x_bot| >= 1), x_top - floor(x_bot) = fr + D_e and floor(x_bot) - x_top = D_e - fr
+  // exactly, and 1/D_e is fp_row_add's reciprocal of D_e (from the table) -- so each row's
+  // fraction, and the sum, are bit for bit fp_row_add's.  The wave's x_bot extremes over the
+  // block are at its extreme rays and rows (x_bot is monotone in the intercept and the row).
+  const float ic0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, icpt), 0));
+  const float ic1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, icpt), 63));
+  const float ic_lo = fminf(ic0, ic1), ic_hi = fmaxf(ic0, ic1);
+  auto block_step = [&](float tab, float2& dr) -> bool {
+    const float t0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tab), 0));
+    const float t1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tab), ROWS - 1));
+    const int x0 = __builtin_amdgcn_readfirstlane(floor_to_int(fminf(t0, t1) + ic_lo));
+    const int x1 = __builtin_amdgcn_readfirstlane(floor_to_int(fmaxf(t0, t1) + ic_hi));
+    const int e = tile_binade(x0, x1);
+    if (e < 0 || e >= FP_STEP_BINADES) return false;
+    dr = steps[e];
+    return dr.x > 0.0f;
+  };
   // The LDS reads run PF rows ahead of the sums (rolling window of PF row registers).
+  // xbs: the row's x_bot (per-row step) or fract(x_bot) (exact step).
   float xbs[PF], vas[PF], vcs[PF];
-  auto fetch = [&](auto kc, float tab, int bbase) {
+  auto fetch = [&](auto ex, auto kc, float tab, int bbase) {
     constexpr int K = decltype(kc)::value;
     constexpr int S = K % PF;
-    xbs[S] = bcast16<K>(tab) + icpt;
-    // LDS byte address of the pair's first column in row slot 0
-    const unsigned off = (unsigned)(__mul24(floor_to_int(xbs[S]), 4 * CS) + bbase);
+    const float xb = bcast16<K>(tab) + icpt;
+    // LDS byte address of the pair's first column in row slot 0: 4*CS*floor(xbot) + bbase
+    unsigned off;
+    if constexpr (decltype(ex)::value) {
+      xbs[S] = __builtin_amdgcn_fractf(xb);
+      // floor(xbot) = xbot - fract(xbot) exactly; the address in FP32 is exact (integers
+      // below 2^24)
+      off = (unsigned)(int)fmaf(xb - xbs[S], (float)(4 * CS), (float)bbase);
+    } else {
+      xbs[S] = xb;
+      off = (unsigned)(__mul24(floor_to_int(xb), 4 * CS) + bbase);
+    }
     const lds_float* p = (const lds_float*)(uintptr_t)off;
     vas[S] = p[K];
     vcs[S] = p[K + CS];
   };
-  auto sum_row = [&](int S) {
-    const float xb = xbs[S], xt = xb + slope;
-    acc = POS ? fp_row_add<true, NT>(acc, xb, xt, vas[S], vcs[S], rs)
-              : fp_row_add<false, NT>(acc, xb, xt, vcs[S], vas[S], rs);
+  auto sum_row = [&](auto ex, int S, float2 dr) {
+    if constexpr (decltype(ex)::value) {
+      const float fr = xbs[S], D = dr.x, rD = dr.y;
+      float fo = POS ? fmaf(fr + D, rD, -rD) : (D - fr) * rD;
+      fo = fminf(fmaxf(fo, 0.0f), 1.0f);
+      const float vb = POS ? vas[S] : vcs[S], vo = POS ? vcs[S] : vas[S];
+      acc = fmaf(fo, vo - vb, acc + vb);
+    } else {
+      const float xb = xbs[S], xt = xb + slope;
+      acc = POS ? fp_row_add<true, NT>(acc, xb, xt, vas[S], vcs[S], rs)
+                : fp_row_add<false, NT>(acc, xb, xt, vcs[S], vas[S], rs);
+    }
   };
-  auto step = [&](auto kc, float tab, int bbase) {  // sum row K, then fetch row K + PF
+  auto step = [&](auto ex, auto kc, float tab, int bbase, float2 dr) {  // sum row K, fetch K + PF
     constexpr int K = decltype(kc)::value;
-    sum_row(K);
-    fetch(std::integral_constant<int, K + PF>{}, tab, bbase);
+    sum_row(ex, K, dr);
+    fetch(ex, std::integral_constant<int, K + PF>{}, tab, bbase);
     __builtin_amdgcn_sched_barrier(0);
   };
-  auto compute = [&](int buf, float tab, int sb) {
+  auto compute = [&](auto ex, int buf, float tab, int sb, float2 dr) {
     // LDS byte address of the pair's first column in the block's row 0: 4*CS*floor(xbot) + bbase
     const int bbase = (buf * W * CS + (h - sb - (POS ? 0 : 1)) * CS) * 4 + seg_lds;
-    unroll_seq(std::make_integer_sequence<int, PF>{}, [&](auto kc) { fetch(kc, tab, bbase); });
+    unroll_seq(std::make_integer_sequence<int, PF>{}, [&](auto kc) { fetch(ex, kc, tab, bbase); });
     __builtin_amdgcn_sched_barrier(0);
-    unroll_seq(std::make_integer_sequence<int, ROWS - PF>{}, [&](auto kc) { step(kc, tab, bbase); });
+    unroll_seq(std::make_integer_sequence<int, ROWS - PF>{}, [&](auto kc) { step(ex, kc, tab, bbase, dr); });
 #pragma unroll
-    for (int S = 0; S < PF; ++S) sum_row(S);
+    for (int S = 0; S < PF; ++S) sum_row(ex, S, dr);
   };
   float tab, tabn;
   int sb, sbn;
@@ -964,7 +1010,13 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
     }
     __builtin_amdgcn_sched_barrier(0);  // the next block's loads stay ahead of the compute
     // A wave skips blocks outside its own rays' row range: there they add exactly 0.
-    if (ib + ROWS > i0w && ib < i1w) compute(NBUF == 2 ? (b & 1) : 0, tab, sb);
+    if (ib + ROWS > i0w && ib < i1w) {
+      float2 dr = make_float2(0.0f, 0.0f);
+      if (steps && block_step(tab, dr))
+        compute(std::true_type{}, NBUF == 2 ? (b & 1) : 0, tab, sb, dr);
+      else
+        compute(std::false_type{}, NBUF == 2 ? (b & 1) : 0, tab, sb, dr);
+    }
     if constexpr (NBUF == 1) __syncthreads();
     if (more) store(NBUF == 2 ? ((b + 1) & 1) : 0);
     __syncthreads();
@@ -1061,7 +1113,8 @@ __device__ __forceinline__ int fpg_line(const int4& g, int k) {
   return k == 0 ? (g.x & 0xffffff) : (k == 1 ? g.y : (k == 2 ? g.z : g.w));
 }
 
-template <int MODE, int NBUF, int CPL>
+// XSTEP: take the exact per-block step where the table allows (off: the per-row step only).
+template <int MODE, int NBUF, int CPL, bool XSTEP = true>
 __device__ __forceinline__ void fpg_item(const FpParams& P, const int4 grp, int blk, int bz, float* seg) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1112,18 +1165,20 @@ __device__ __forceinline__ void fpg_item(const FpParams& P, const int4 grp, int 
     return sb;
   };
   const float* src = ((L.flags & LINE_FLAT) ? P.imgT : P.img) + (size_t)bz * N * N;
+  const float2* lsteps =
+      XSTEP && P.steps ? P.steps + (size_t)fpg_line(grp, active ? wave : 0) * FP_STEP_BINADES : nullptr;
   float acc = 0.0f;
   if (i0 < i1) {
     const bool nt = L.flags & LINE_NEWTON;
     const float rs = nt ? 1.0f / fabsf(slope) : 0.0f;
     if ((L.flags & LINE_SLOPE_POS) && nt)
-      acc = fps_walk<true, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<true, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs, lsteps);
     else if (L.flags & LINE_SLOPE_POS)
-      acc = fps_walk<true, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<true, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, 0.0f, lsteps);
     else if (nt)
-      acc = fps_walk<false, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs);
+      acc = fps_walk<false, NBUF, true, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, rs, lsteps);
     else
-      acc = fps_walk<false, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg);
+      acc = fps_walk<false, NBUF, false, CPL, FPG_ROWS>(src, N, h, i0, i1, i0w, i1w, slope, icpt, seg_start, seg, 0.0f, lsteps);
   }
   if (!active || r >= P.R) return;  // after the walk: every lane took part in its barriers
   const float s = acc * L.lbase;

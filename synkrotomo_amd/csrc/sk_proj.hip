@@ -129,7 +129,7 @@ int launch_transpose(Ctx* c, const float* in, float* out, int N, int batch, hipS
 
 FpParams fp_params(const Geom* g, const FpArgs& a) {
   return FpParams{g->d_lines, g->A, g->R, g->N, g->ncand, g->rho0, g->drho,
-                  a.img, a.imgT, a.out, a.b, a.rowinv, g->d_fpg};
+                  a.img, a.imgT, a.out, a.b, a.rowinv, g->d_fpg, g->d_fp_steps};
 }
 
 // fp variants: 0/1/2 = 1/2/4 rays per lane, bin blocks fastest; 3/4 = 1/2 rays per lane,

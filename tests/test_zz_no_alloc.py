@@ -34,7 +34,8 @@ def _err(geo):
     return nat.lib().sk_last_error(geo.ctx.handle).decode()
 
 
-def test_fp_noalloc_equals_fp_twice_second_smaller():
+@pytest.mark.parametrize("fp_variant", [-1, 7], ids=["default", "grouped"])
+def test_fp_noalloc_equals_fp_twice_second_smaller(fp_variant):
     from synkrotomo_amd.projector import Context
     ctx = Context(0)  # a fresh context: its own scratch has never been grown
     try:
@@ -43,6 +44,7 @@ def test_fp_noalloc_equals_fp_twice_second_smaller():
         prepared = []
         for N, A, R, rz, b in cases:
             geo = ctx.geometry(_uniform(A), rz, 1.0, R, N)
+            geo.set_variants(fp_variant, -1)
             x = torch.rand(b, N, N, device="cuda", generator=gen)
             prepared.append((geo, x, torch.full((b, A, R), float("nan"), device="cuda")))
         # The HIP runtime sets up per-kernel state (argument buffers and the like) on a kernel's
