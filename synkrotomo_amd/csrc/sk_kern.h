@@ -932,14 +932,23 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   // exactly, and 1/D_e is fp_row_add's reciprocal of D_e (from the table) -- so each row's
   // fraction, and the sum, are bit for bit fp_row_add's.  The wave's x_bot extremes over the
   // block are at its extreme rays and rows (x_bot is monotone in the intercept and the row).
+  // The block's column range from 16.16 fixed-point bounds on the scalar unit: x_bot =
+  // RN(RN(i*slope) + icpt) lies within 5e-4 of i*slope + icpt (|x| < 2^12), the fixed-point
+  // slope and intercepts within 2^-17*|i| + 2^-16 of theirs, so the floors of the fixed-point
+  // extremes, widened by one column each way, bound every lane's and row's floor(x_bot).
+  // Walks with |intercept| >= 2^14 or images over 8192 rows (where the products could leave
+  // 32 bits) take the per-row step.
   const float ic0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, icpt), 0));
   const float ic1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, icpt), 63));
   const float ic_lo = fminf(ic0, ic1), ic_hi = fmaxf(ic0, ic1);
-  auto block_step = [&](float tab, float2& dr) -> bool {
-    const float t0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tab), 0));
-    const float t1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tab), ROWS - 1));
-    const int x0 = __builtin_amdgcn_readfirstlane(floor_to_int(fminf(t0, t1) + ic_lo));
-    const int x1 = __builtin_amdgcn_readfirstlane(floor_to_int(fmaxf(t0, t1) + ic_hi));
+  const bool xok = steps && N <= 8192 && ic_lo > -16384.0f && ic_hi < 16384.0f;
+  const int s_fx = __builtin_amdgcn_readfirstlane(xok ? (int)rintf(slope * 65536.0f) : 0);
+  const int lo_fx = __builtin_amdgcn_readfirstlane(xok ? (int)floorf(ic_lo * 65536.0f) : 0);
+  const int hi_fx = __builtin_amdgcn_readfirstlane(xok ? (int)ceilf(ic_hi * 65536.0f) : 0);
+  auto block_step = [&](int ib, float2& dr) -> bool {
+    if (!xok) return false;
+    const int a = ib * s_fx, z = (ib + ROWS - 1) * s_fx;
+    const int x0 = ((min(a, z) + lo_fx) >> 16) - 1, x1 = ((max(a, z) + hi_fx) >> 16) + 1;
     const int e = tile_binade(x0, x1);
     if (e < 0 || e >= FP_STEP_BINADES) return false;
     dr = steps[e];
@@ -1012,7 +1021,7 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
     // A wave skips blocks outside its own rays' row range: there they add exactly 0.
     if (ib + ROWS > i0w && ib < i1w) {
       float2 dr = make_float2(0.0f, 0.0f);
-      if (steps && block_step(tab, dr))
+      if (block_step(ib, dr))
         compute(std::true_type{}, NBUF == 2 ? (b & 1) : 0, tab, sb, dr);
       else
         compute(std::false_type{}, NBUF == 2 ? (b & 1) : 0, tab, sb, dr);
