@@ -877,7 +877,7 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   constexpr int RPW = ROWS / (FPS_RAYS / 64);  // staged rows per wave
   constexpr int CS = ROWS + 1;                 // LDS stride of a staged column (floats)
   constexpr int PF = ROWS / 2;                 // rows of LDS reads in flight ahead of the sums
-  constexpr int W = 64 * CPL;                      // staged columns
+  constexpr int W = 64 * CPL;                  // staged columns
   float st[RPW][CPL];
   // Per block, lane l16 holds row ib + l16's i*slope (tab); the block's segment starts at
   // image column sb = seg_start(ib, tab).
@@ -933,9 +933,10 @@ __device__ __forceinline__ float fps_walk(const float* src, int N, int h, int i0
   // fraction, and the sum, are bit for bit fp_row_add's.  The wave's x_bot extremes over the
   // block are at its extreme rays and rows (x_bot is monotone in the intercept and the row).
   // The block's column range from 16.16 fixed-point bounds on the scalar unit: x_bot =
-  // RN(RN(i*slope) + icpt) lies within 5e-4 of i*slope + icpt (|x| < 2^12), the fixed-point
-  // slope and intercepts within 2^-17*|i| + 2^-16 of theirs, so the floors of the fixed-point
-  // extremes, widened by one column each way, bound every lane's and row's floor(x_bot).
+  // RN(RN(i*slope) + icpt) lies within 3e-3 of i*slope + icpt (|i| <= 4112, |icpt| < 2^14),
+  // the fixed-point products and intercepts within 2^-17*|i| + 2^-16 (< 0.04) of theirs, so
+  // the floors of the fixed-point extremes, widened by one column each way, bound every lane's
+  // and row's floor(x_bot).
   // Walks with |intercept| >= 2^14 or images over 8192 rows (where the products could leave
   // 32 bits) take the per-row step.
   const float ic0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, icpt), 0));
