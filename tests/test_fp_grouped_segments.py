@@ -145,3 +145,86 @@ def test_the_bound_needs_the_block_drift():
     wrong = group_table(lines, ns, N, R, rho0, drho, drift_rows=0)
     used = [u for _, _, u, _ in reads(angles, N, R, rho0, drho)]
     assert any(w < u for (_, _, w), u in zip(wrong, used))
+
+
+# ------------------------------------------------------------ exact per-block step (fps_walk)
+FP_STEP_BINADES = 16
+
+
+def tile_binade(x0, x1):
+    lo, hi = x0 - 4, x1 + 5
+    if lo <= 0 <= hi:
+        return -1
+    a, b = (lo, hi) if lo > 0 else (-hi, -lo)
+    ea, eb = a.bit_length() - 1, b.bit_length() - 1
+    return ea if ea == eb else -1
+
+
+def binade_step(slope, e):
+    t = np.ldexp(abs(f32(slope)), 23 - e).astype(f32)
+    if t - np.floor(t) == f32(0.5):
+        return f32(0.0)
+    return np.ldexp(np.rint(t), e - 23).astype(f32)
+
+
+def exact_blocks(angles, N, R, rho0, drho, rows=ROWS):
+    """Every (wave, block) the scalar-unit decision sends down the exact step, with the check
+    that it may: each lane's and row's x_bot and x_top lie in the binade and differ by exactly
+    D_e, and x_top - floor(x_bot), floor(x_bot) - x_top equal fract(x_bot) + D_e, D_e -
+    fract(x_bot) in float32.  Returns (exact blocks, all blocks)."""
+    lines, ns = frame_lines(angles, N)
+    rho0, drho = f32(rho0), f32(drho)
+    table = group_table(lines, ns, N, R, float(rho0), float(drho), rows)
+    h, nb = N // 2, (R + BINS - 1) // BINS
+    n_exact = n_all = 0
+    for grp, _, _ in table:
+        for blk in range(nb):
+            first, last = blk * BINS, min(blk * BINS + BINS - 1, R - 1)
+            r = np.arange(first, first + BINS, dtype=f32)
+            hull = []
+            icpts = [((rho0 + r * drho) / lines[j][1]).astype(f32) for j in grp]
+            for j, icp in zip(grp, icpts):
+                hull += [row_clip(lines[j][2], icp[0], h),
+                         row_clip(lines[j][2], f32((rho0 + f32(last) * drho) / lines[j][1]), h)]
+            i0, i1 = min(a for a, _ in hull), max(b for _, b in hull)
+            for j, icp in zip(grp, icpts):
+                slope = lines[j][2]
+                ic_lo, ic_hi = min(icp[0], icp[63]), max(icp[0], icp[63])
+                if not (N <= 8192 and ic_lo > -16384 and ic_hi < 16384):
+                    continue
+                s_fx = int(np.rint(slope * f32(65536.0)))
+                lo_fx, hi_fx = int(np.floor(ic_lo * f32(65536.0))), int(np.ceil(ic_hi * f32(65536.0)))
+                for ib in range(i0, i1, rows):
+                    n_all += 1
+                    a, z = ib * s_fx, (ib + rows - 1) * s_fx
+                    e = tile_binade(((min(a, z) + lo_fx) >> 16) - 1, ((max(a, z) + hi_fx) >> 16) + 1)
+                    if e < 0 or e >= FP_STEP_BINADES:
+                        continue
+                    D = binade_step(slope, e)
+                    if not D > 0:
+                        continue
+                    n_exact += 1
+                    tab = (np.arange(ib, ib + rows).astype(f32) * slope).astype(f32)
+                    xb = (tab[:, None] + icp[None, :]).astype(f32)
+                    xt = (xb + slope).astype(f32)
+                    for x in (xb, xt):
+                        m = np.abs(x)
+                        assert np.all(m >= 2.0**e) and np.all(m < 2.0**(e + 1)), (j, ib, e)
+                        assert np.all(np.sign(x) == np.sign(xb[0, 0]))
+                    assert np.all(np.abs((xt - xb).astype(f32)) == D)
+                    fx = np.floor(xb).astype(f32)
+                    fr = (xb - fx).astype(f32)
+                    if slope >= 0:
+                        assert np.array_equal((xt - fx).astype(f32), (fr + D).astype(f32))
+                    else:
+                        assert np.array_equal((fx - xt).astype(f32), (D - fr).astype(f32))
+    return n_exact, n_all
+
+
+@pytest.mark.parametrize("name,angles,N,R,rho0,drho", CASES[:2] + CASES[3:4],
+                         ids=[CASES[0][0], CASES[1][0], CASES[3][0]])
+def test_exact_step_blocks_are_exact(name, angles, N, R, rho0, drho):
+    n_exact, n_all = exact_blocks(angles, N, R, rho0, drho)
+    assert n_all > 0
+    if N >= 256:
+        assert n_exact > 0  # the step applies somewhere (binades wide enough for a block)
