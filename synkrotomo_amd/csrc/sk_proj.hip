@@ -18,7 +18,7 @@
 //   k_fp_grouped A.x, grouped staged (variant 7): a workgroup = 64 consecutive bins of four
 //               lines adjacent in slope, one wave per line, so the staged segment of a
 //               block is two to four chunks of 64 columns instead of six; blocks of 16
-//               rows.  Same walk.
+//               rows.  Same walk.  k_fp_grouped_res: its SIRT-residual instance.
 //   k_bp  B.r   (futhark/backprojection.fut:20-93).  Pixel-parallel gather, no atomics: a
 //               block owns a 16 x 16*PPT pixel tile and walks the lines in order, staging
 //               for each block of 32 lines the tile's rho-window of the sinogram in LDS
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(FPS_RAYS) void k_fp_staged(FpParams P) {
 // Grouped staged A.x (variant 7): grid (line groups, bin blocks of FPG_BINS, images).
 // Dynamic LDS: NBUF * (FPG_ROWS + 1) * 64 * (the geometry's largest CPL) floats.
 template <int MODE, int NBUF>
-__global__ __launch_bounds__(FPS_RAYS, 8) void k_fp_grouped(FpParams P) {
+__device__ __forceinline__ void fpg_block(const FpParams& P) {
   extern __shared__ float4 seg4[];
   const int4 grp = P.groups[blockIdx.x];
   const int cpl = grp.x >> 24;  // uniform
@@ -98,6 +98,21 @@ __global__ __launch_bounds__(FPS_RAYS, 8) void k_fp_grouped(FpParams P) {
     fpg_item<MODE, NBUF, 3>(P, grp, blockIdx.y, blockIdx.z, (float*)seg4);
   else
     fpg_item<MODE, NBUF, 4>(P, grp, blockIdx.y, blockIdx.z, (float*)seg4);
+}
+
+template <int MODE, int NBUF>
+__global__ __launch_bounds__(FPS_RAYS, 8) void k_fp_grouped(FpParams P) {
+  fpg_block<MODE, NBUF>(P);
+}
+
+// The same for the SIRT residual (FP_SIRT), whose epilogue keeps two more pointers across the
+// walk: under the 8-workgroups-per-CU bound's scalar-register budget (about 80) they spilled
+// 14 scalar registers into a vector register's lanes and one vector register to scratch,
+// reloaded once per row block.  64 vector registers (still 8 waves per SIMD) without that
+// bound's scalar limit: 84 scalar registers, no spill.
+template <int NBUF>
+__global__ __launch_bounds__(FPS_RAYS) __attribute__((amdgpu_num_vgpr(64))) void k_fp_grouped_res(FpParams P) {
+  fpg_block<FP_SIRT, NBUF>(P);
 }
 
 // Grid: (tiles across, tiles down, images).  Dynamic LDS: BP_LB * W float2.
@@ -176,8 +191,12 @@ template <int MODE, int NBUF>
 static void fp_go_grouped(const Geom* g, const FpArgs& a, hipStream_t s) {
   const unsigned nb = (unsigned)((g->R + FPG_BINS - 1) / FPG_BINS);
   const size_t lds = sizeof(float) * NBUF * (FPG_ROWS + 1) * 64 * (size_t)g->fpg_cpl;
-  hipLaunchKernelGGL((k_fp_grouped<MODE, NBUF>), dim3(g->n_fpg, nb, a.batch), dim3(FPS_RAYS), lds, s,
-                     fp_params(g, a));
+  if constexpr (MODE == FP_SIRT)
+    hipLaunchKernelGGL((k_fp_grouped_res<NBUF>), dim3(g->n_fpg, nb, a.batch), dim3(FPS_RAYS), lds, s,
+                       fp_params(g, a));
+  else
+    hipLaunchKernelGGL((k_fp_grouped<MODE, NBUF>), dim3(g->n_fpg, nb, a.batch), dim3(FPS_RAYS), lds, s,
+                       fp_params(g, a));
 }
 
 int launch_fp(Ctx* c, const Geom* g, FpMode mode, const FpArgs& a, hipStream_t s) {
