@@ -136,9 +136,13 @@ static std::vector<unsigned> bp_tile_orders(int N, int n_steep, int A, size_t of
     t.reserve((size_t)ntx * nty);
     for (int ty = 0; ty < nty; ++ty)
       for (int tx = 0; tx < ntx; ++tx) {
+        // per slot: about 61 VALU per line and lane with the exact step, 100 without
         const int Y64 = (ty * TH) & ~63;
-        const long cs = tile_binade(tx * BP_TW - h, tx * BP_TW - h + BP_TW - 1) >= 0 ? 61 : 100;
-        const long cf = tile_binade(Y64 - h, Y64 - h + 63) >= 0 ? 61 : 100;
+        const long cs = (tile_binade(tx * BP_TW - h, tx * BP_TW - h + BP_TW - 1) >= 0 ? 61 : 100) * ppts[i];
+        int ef, band;
+        bp_band_rule(Y64 - h, ef, band);
+        const long nex = __builtin_popcount((unsigned)((band >> ((ty * TH - Y64) >> 4)) & ((1 << ppts[i]) - 1)));
+        const long cf = 61 * nex + 100 * (ppts[i] - nex);
         t.push_back({-(cs * n_steep + cf * (A - n_steep)), ((unsigned)ty << 16) | (unsigned)tx});
       }
     std::stable_sort(t.begin(), t.end(),

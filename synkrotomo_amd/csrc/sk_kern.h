@@ -435,6 +435,27 @@ __host__ __device__ inline int tile_binade(int x0, int x1) {
   return ea == eb ? ea : -1;
 }
 
+// Flat lines' exact-step rule for the 64-row band whose first frame row is y0: e = the binade
+// of its first 16-row sub-band that lies in one (tile_binade), mask bit s = sub-band s lies in
+// binade e.  Only the masks a band starting at a multiple of 64 can have -- 1111, and 0111,
+// 1110, 0110 where a power of two lies at an edge of the band -- are kept; any other gives
+// e = -1, mask 0.  Every tile shape derives its pixels' choice from the same band, so the
+// variants stay bitwise equal.
+__host__ __device__ inline void bp_band_rule(int y0, int& e, int& mask) {
+  e = -1;
+  mask = 0;
+  for (int s = 0; s < 4; ++s) {
+    const int es = tile_binade(y0 + 16 * s, y0 + 16 * s + 15);
+    if (es < 0) continue;
+    if (e < 0) e = es;
+    if (es == e) mask |= 1 << s;
+  }
+  if (mask != 15 && mask != 7 && mask != 14 && mask != 6) {
+    e = -1;
+    mask = 0;
+  }
+}
+
 // The exact step d = |fl(x_bot + slope) - x_bot| of every x_bot in binade e whose x_top stays
 // in it: x_bot is a multiple of u = 2^(e-23), so fl(x_bot + slope) = x_bot + RN_u(slope), and d
 // = RN_u(|slope|) whatever x_bot is -- unless |slope| lies exactly halfway between two multiples
@@ -485,13 +506,17 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
     acc_f[k] = 0.0f;
   }
 
-  // hoisted fast path: the tile's binade for steep lines (frame x = image column) and for flat
-  // lines (frame x = image row); -1 where the tile straddles a power of two or zero.  The
-  // rows are taken 64 at a time whatever the tile height, so every tile shape makes the same
-  // choice for a pixel and the variants stay bitwise equal.
+  // hoisted fast path: the tile's binade for steep lines (frame x = image column; -1 where
+  // the tile's columns straddle a power of two or zero) and for flat lines (frame x = image
+  // row): the binade of the 64-row band holding the tile (bp_band_rule), whose 16-row
+  // sub-bands in it take the exact step -- fslots bit q: the sub-band of slot q (rows
+  // Y0 + 16q ..).  The other slots of such a tile take the per-candidate step scaled by the
+  // step (their staged values carry lbase/dx as well).
   const int Y64 = Y0 & ~63;
-  const int ebin[2] = {REV ? tile_binade(X0 - h, X0 - h + BP_TW - 1) : -1,
-                       REV ? tile_binade(Y64 - h, Y64 - h + 63) : -1};
+  int eflat = -1, fband = 0;
+  if (REV) bp_band_rule(Y64 - h, eflat, fband);
+  const int fslots = (fband >> ((Y0 - Y64) >> 4)) & ((1 << PPT) - 1);
+  const int ebin[2] = {REV ? tile_binade(X0 - h, X0 - h + BP_TW - 1) : -1, eflat};
 
   for (int l0 = 0; l0 < P.A; l0 += BP_LB) {
     const int nl = min(BP_LB, P.A - l0);
@@ -582,8 +607,13 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
         // has the sign and the degenerate case resolved at compile time (no per-line branch).
         // Kinds: 0: slope >= 0, 1: slope < 0, 2: degenerate; 3, 4: as 0, 1 on LINE_NEWTON
         // lines; 5, 6: as 0, 1 in a tile with one binade (tile_binade), with the exact step.
-        auto line_body = [&](auto kind, int jj) {
+        // SLOTS (kinds 0, 1, 3, 4 of a flat run whose band has a binade, bp_band_rule): bit q
+        // set: slot q takes the exact step as kinds 5, 6 do; the other slots take the kind's
+        // own step, times dx (the staged value is lbase/dx).  -1: the kind's step, unscaled.
+        auto line_body = [&](auto kind, auto slots, int jj) {
           constexpr int KIND = decltype(kind)::value;
+          constexpr int SLOTS = decltype(slots)::value;
+          constexpr bool POS = KIND == 0 || KIND == 3 || KIND == 5;
           const BpLineLds lc = lsh[jj];
           const float slope = lc.slope;
           // steep run: frame x (the image column) is shared by a lane's pixels; flat run: y.
@@ -595,8 +625,10 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
           const float2* wrev = win + __builtin_amdgcn_readfirstlane(jj * W + W - NC + lc.w0);
           float2 ev[PPT][NC];
           float xbase[PPT], edge[PPT];
+          bool ex[PPT];  // the slot takes the exact step (constant once the loops unroll)
 #pragma unroll
           for (int q = 0; q < PPT; ++q) {
+            ex[q] = KIND == 5 || KIND == 6 || (KIND != 2 && SLOTS >= 0 && ((SLOTS >> q) & 1));
             // t is within 0.4 bin of the reference's index (|rho0| + N < 2^20 for this
             // path), so its ceiling lies in [w0, w0 + W - NC]: no clamp.
             const float t = part == 0 ? fmaf(lc.si, yh[q], tb) : fmaf(lc.ci, xh[q], tb);
@@ -605,8 +637,7 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
             // x_top - xmin = x_bot - (xmin - d), resp. xmin + 1 - x_top = (xmin + 1 + d) -
             // x_bot, exactly: x_bot, x_top and the moved edge are multiples of u in the
             // tile's binade
-            if constexpr (KIND == 5) edge[q] = xm[q] - lc.dx;
-            if constexpr (KIND == 6) edge[q] = x1[q] + lc.dx;
+            if (ex[q]) edge[q] = POS ? xm[q] - lc.dx : x1[q] + lc.dx;
 #pragma unroll
             for (int k = 0; k < NC; ++k) ev[q][k] = wrow[NC - 1 - k];
           }
@@ -618,61 +649,89 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
               const float xb = xbase[q] + ev[q][k].x;
-              const float xt = xb + slope;
-              if constexpr (KIND == 0) {
-                const float m = __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f);
-                const float f = fminf(fmaxf(m * fast_rcp(xt - xb), 0.0f), 1.0f);
-                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-              } else if constexpr (KIND == 1) {
-                const float m = __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
-                const float f = fminf(fmaxf(m * fast_rcp(xb - xt), 0.0f), 1.0f);
-                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-              } else if constexpr (KIND == 3 || KIND == 4) {
-                // 1/d by one Newton step from 1/|slope| (d = |x_top - x_bot| is |slope| to
-                // within the rounding of x_bot): two full-rate FMAs for the quarter-rate
-                // v_rcp_f32
-                const float d = KIND == 3 ? xt - xb : xb - xt;
-                const float m = KIND == 3 ? __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f)
-                                          : __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
-                const float r = fmaf(lc.rs, fmaf(-d, lc.rs, 1.0f), lc.rs);
-                const float f = fminf(fmaxf(m * r, 0.0f), 1.0f);
-                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
-              } else if constexpr (KIND == 5 || KIND == 6) {
-                // min(m/d, 1) = min(m, d)/d with d = dx exact and 1/d in the staged value:
-                // max(min(a, b, d), 0) is one v_min3 with the output clamp (d <= 1).  A line
-                // whose step is a rounding tie has dx = 0 and adds 0 here (tie pass below).
-                const float a = KIND == 5 ? xb - edge[q] : xb - xm[q];
-                const float b = KIND == 5 ? x1[q] - xb : edge[q] - xb;
-                const float g = __builtin_amdgcn_fmed3f(fminf(fminf(a, b), lc.dx), 0.0f, 1.0f);
-                acc[q] = fmaf(g, ev[q][k].y, acc[q]);
-              } else {
+              if constexpr (KIND == 2) {
                 // degenerate line: general form with the m == 0 case, lbase already in e.y
                 const float l = cand_len<false, W_REF>(xb, slope, 1.0f, xm[q], x1[q], true);
                 acc[q] = fmaf(l, ev[q][k].y, acc[q]);
+              } else if (ex[q]) {
+                // min(m/d, 1) = min(m, d)/d with d = dx exact and 1/d in the staged value:
+                // max(min(a, b, d), 0) is one v_min3 with the output clamp (d <= 1).  A line
+                // whose step is a rounding tie has dx = 0 and adds 0 here (tie pass below).
+                const float a = POS ? xb - edge[q] : xb - xm[q];
+                const float b = POS ? x1[q] - xb : edge[q] - xb;
+                const float g = __builtin_amdgcn_fmed3f(fminf(fminf(a, b), lc.dx), 0.0f, 1.0f);
+                acc[q] = fmaf(g, ev[q][k].y, acc[q]);
+              } else {
+                const float xt = xb + slope;
+                const float d = POS ? xt - xb : xb - xt;
+                const float m = POS ? __builtin_amdgcn_fmed3f(xt - xm[q], x1[q] - xb, 0.0f)
+                                    : __builtin_amdgcn_fmed3f(xb - xm[q], x1[q] - xt, 0.0f);
+                float f;
+                if constexpr (KIND == 3 || KIND == 4) {
+                  // 1/d by one Newton step from 1/|slope| (d = |x_top - x_bot| is |slope| to
+                  // within the rounding of x_bot): two full-rate FMAs for the quarter-rate
+                  // v_rcp_f32
+                  const float r = fmaf(lc.rs, fmaf(-d, lc.rs, 1.0f), lc.rs);
+                  f = fminf(fmaxf(m * r, 0.0f), 1.0f);
+                } else {
+                  f = fminf(fmaxf(m * fast_rcp(d), 0.0f), 1.0f);
+                }
+                // a tie line (dx = 0) adds 0 here as well
+                if constexpr (SLOTS >= 0) f = f * lc.dx;
+                acc[q] = fmaf(f, ev[q][k].y, acc[q]);
               }
             }
         };
-        auto run = [&](auto kind, int jb, int je) {
-          for (int jj = jb; jj < je; ++jj) line_body(kind, jj);
+        auto run = [&](auto kind, auto slots, int jb, int je) {
+          for (int jj = jb; jj < je; ++jj) line_body(kind, slots, jj);
+        };
+        using none = std::integral_constant<int, -1>;
+        // a flat run in a tile whose band has a binade but not every slot in it: the slot
+        // masks each tile height can meet (bp_band_rule's bands)
+        auto run_mixed = [&](auto kind, int jb, int je) {
+          if constexpr (PPT == 4) {
+            if (fslots == 7)
+              run(kind, std::integral_constant<int, 7>{}, jb, je);
+            else if (fslots == 14)
+              run(kind, std::integral_constant<int, 14>{}, jb, je);
+            else
+              run(kind, std::integral_constant<int, 6>{}, jb, je);
+          } else if constexpr (PPT == 2) {
+            if (fslots == 1)
+              run(kind, std::integral_constant<int, 1>{}, jb, je);
+            else
+              run(kind, std::integral_constant<int, 2>{}, jb, je);
+          } else {
+            run(kind, std::integral_constant<int, 0>{}, jb, je);
+          }
         };
         const bool exact_step = ebin[part] >= 0;  // uniform: the tile's binade for this part
+        const bool all_slots = part == 0 || fslots == (1 << PPT) - 1;  // uniform
         for (int jr = j_begin; jr < j_end;) {  // uniform: scalar loop over the runs
           const int f = P.lines[l0 + jr].flags;
           const int je = min(jr + (f >> LINE_RUN_SHIFT), j_end);
           if (f & LINE_DEGEN)
-            run(std::integral_constant<int, 2>{}, jr, je);
+            run(std::integral_constant<int, 2>{}, none{}, jr, je);
+          else if (exact_step && all_slots && (f & LINE_SLOPE_POS))
+            run(std::integral_constant<int, 5>{}, none{}, jr, je);
+          else if (exact_step && all_slots)
+            run(std::integral_constant<int, 6>{}, none{}, jr, je);
+          else if (exact_step && (f & LINE_NEWTON) && (f & LINE_SLOPE_POS))
+            run_mixed(std::integral_constant<int, 3>{}, jr, je);
+          else if (exact_step && (f & LINE_NEWTON))
+            run_mixed(std::integral_constant<int, 4>{}, jr, je);
           else if (exact_step && (f & LINE_SLOPE_POS))
-            run(std::integral_constant<int, 5>{}, jr, je);
+            run_mixed(std::integral_constant<int, 0>{}, jr, je);
           else if (exact_step)
-            run(std::integral_constant<int, 6>{}, jr, je);
+            run_mixed(std::integral_constant<int, 1>{}, jr, je);
           else if ((f & LINE_NEWTON) && (f & LINE_SLOPE_POS))
-            run(std::integral_constant<int, 3>{}, jr, je);
+            run(std::integral_constant<int, 3>{}, none{}, jr, je);
           else if (f & LINE_NEWTON)
-            run(std::integral_constant<int, 4>{}, jr, je);
+            run(std::integral_constant<int, 4>{}, none{}, jr, je);
           else if (f & LINE_SLOPE_POS)
-            run(std::integral_constant<int, 0>{}, jr, je);
+            run(std::integral_constant<int, 0>{}, none{}, jr, je);
           else
-            run(std::integral_constant<int, 1>{}, jr, je);
+            run(std::integral_constant<int, 1>{}, none{}, jr, je);
           jr = je;
         }
         if (exact_step) {
@@ -685,9 +744,9 @@ __device__ __forceinline__ void bp_item(const BpParams& P, int tile_x, int tile_
             const int jj = __builtin_ctzll(tm);
             tm &= tm - 1;
             if (P.lines[l0 + jj].flags & LINE_SLOPE_POS)
-              line_body(std::integral_constant<int, 0>{}, jj);
+              line_body(std::integral_constant<int, 0>{}, none{}, jj);
             else
-              line_body(std::integral_constant<int, 1>{}, jj);
+              line_body(std::integral_constant<int, 1>{}, none{}, jj);
           }
         }
       }

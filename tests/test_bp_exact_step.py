@@ -30,7 +30,8 @@ def binade_step(slope, e):
     return f32(math.ldexp(float(np.rint(t)), e - 23))
 
 
-@pytest.mark.parametrize("N,A,tw,th", [(512, 360, 16, 64), (2048, 128, 16, 64), (256, 90, 16, 16)])
+@pytest.mark.parametrize("N,A,tw,th", [(512, 360, 16, 64), (2048, 128, 16, 64), (256, 90, 16, 16),
+                                        (2048, 128, 16, 16)])
 def test_exact_step_matches_reference_rounding(N, A, tw, th):
     R = int(math.ceil(math.sqrt(2) * N))
     rho0 = f32(-(R - 1) / 2)
