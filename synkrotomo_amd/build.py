@@ -55,7 +55,8 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     probes = sorted(glob.glob(os.path.join(PROBES, "*.hip")))
-    workers = min(len(srcs) + len(probes), max(1, (os.cpu_count() or 4)))
+    # at most 16 compiles at once: a host's core count can exceed its share of it
+    workers = min(len(srcs) + len(probes), 16, max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         probe_jobs = [ex.submit(_probe, p) for p in probes]
         objs = list(ex.map(_compile, srcs))
