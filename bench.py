@@ -332,8 +332,9 @@ def learned_leg(ctx, n_test: int = 128) -> dict:
     Poisson noise I0 = 1e4.  Targets: the literal 1e-2 and 1.1x each phantom's noise floor
     (the 1e-2 target lies below the noise floor at I0 = 1e4, so plain SIRT reaches it only by
     fitting noise, after hundreds of iterations or not within the cap of 1500).  The first
-    n_test phantoms of the seeded test set; their median speedup is the 1,000-phantom one at 128
-    (5.03x on one host), while the first 64 alone sit lower (4.74-5.07x across hosts)."""
+    n_test phantoms of the seeded test set; at 128 their median speedup is about the
+    1,000-phantom one (5.03-5.14x on two hosts against 5.03x), while the first 64 alone sit
+    lower (4.74-5.07x across hosts)."""
     from synkrotomo_amd import learned_run
     if not os.path.exists(learned_run.DEFAULT_WEIGHTS):
         return {"skipped": "no trained weights"}
